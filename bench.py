@@ -1,0 +1,157 @@
+"""Benchmark: Goldilocks n = 2^24 forward NTT on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one forward transform of one resident 2^24-element uint64 buffer
+(uniform in [0, P) from a fixed seed), in place.  value = whole-job
+elements per second over exactly K steps bracketed by device
+synchronisation.  Prints ONE JSON line.
+
+roofline: the dominant kernel (the pass with the largest mean duration),
+timed live with HIP events on the stream it runs on; achieved = its
+algorithmic bytes per launch (each pass reads and writes the whole 2^24 x 8 B
+buffer once: 2 n w = 256 MiB) / its mean duration, against the 8 TB/s HBM3E
+peak.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
+(recorded in profiles/; null here when not supplied).
+
+cpu_baseline: the oracle's restatement of the reference's CPU Stockham
+(src/ntt/ntt_cpu.cpp:46-90 with exact Goldilocks arithmetic), single
+thread, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "Goldilocks 2^24 NTT GElem/s, %HBM peak; batched-vs-plain transforms/s"
+P = 0xFFFFFFFF00000001
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="FETCH_SIZE/WRITE_SIZE per dispatch measured by rocprofv3 (optional)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(log2n: int, budget_s: float) -> dict:
+    """Time the oracle on whole 2^log2n transforms until about budget_s."""
+    from oracle import oracle as O
+    n = 1 << log2n
+    x = np.random.default_rng(7).integers(0, P, size=n, dtype=np.uint64)
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while time.perf_counter() < t_end or not times:
+        t0 = time.perf_counter()
+        O.stockham_gl(x, log2n)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": n / med / 1e9, "unit": "GElem/s", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} whole Goldilocks 2^{log2n} forward transforms, median "
+                      f"{med * 1e3:.0f} ms each, single thread (oracle/ntt_oracle.c)"}
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the transform is not "
+              "sharded across GPUs in this build", file=sys.stderr)
+        return 2
+    import stntt_amd as S
+
+    L = args.log2n
+    n = 1 << L
+    ctx = S.Context(0)
+    x = np.random.default_rng(42).integers(0, P, size=n, dtype=np.uint64)
+    buf = ctx.upload(x)
+    log(f"[bench] goldilocks 2^{L}: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        ctx.ntt_goldilocks(buf, L, 1)
+    ctx.sync()
+
+    # timed region: exactly K steps between two device synchronisations
+    log(f"[bench] timing {args.steps} steps")
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.ntt_goldilocks(buf, L, 1)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e9
+
+    # per-kernel durations with HIP events on the library's stream
+    ctx.set_profiling(True)
+    per_pass: list[list[float]] = []
+    for _ in range(max(args.steps, 10)):
+        ctx.ntt_goldilocks(buf, L, 1)
+        per_pass.append(ctx.profile_read())
+    ctx.set_profiling(False)
+    npass = len(per_pass[0])
+    mean_pass = [statistics.mean(r[i] for r in per_pass) for i in range(npass)]
+    dom = max(range(npass), key=lambda i: mean_pass[i])
+    alg_bytes = 2 * n * 8
+    achieved = alg_bytes / (mean_pass[dom] * 1e-3) / 1e9
+    transform_ms = sum(mean_pass)
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            tj = json.load(open(args.traffic_json))
+            traffic = tj.get(f"goldilocks_2^{L}", {}).get("pass_bytes")
+            if isinstance(traffic, list):
+                traffic = traffic[dom] if dom < len(traffic) else None
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GElem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: uniform uint64 in [0, P) from seed 42, resident in HBM",
+        "config": {"workload": f"goldilocks_forward_n2^{L}_batch1", "field": "goldilocks", "log2n": L,
+                   "batch": 1, "direction": "forward", "passes": npass},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"pass {dom} of {npass}", "kernel_ms": round(mean_pass[dom], 5)},
+        "hbm_frac_transform": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "pass_ms": [round(v, 5) for v in mean_pass],
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)
+    buf.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,122 @@
+/*
+ * stntt.h -- C ABI of the MI355X Stockham NTT library (libstntt.so).
+ *
+ * Forward and inverse number-theoretic transforms, natural order in and out,
+ * over
+ *   - the reference's 32-bit prime field: runtime (root, mod) as produced by
+ *     findParams(n, min_mod), mod < 2^31 prime, root of order exactly n;
+ *   - Goldilocks, P = 2^64 - 2^32 + 1, root 7^((P-1)/n).
+ * Buffers are contiguous [batch][n] rows, transformed in place from the
+ * caller's point of view (src/ntt/ntt_cpu.cpp:87-90,
+ * src/cuda/ntt/ntt_kernel.cuh:159,195 of the reference).
+ *
+ * Error model: every entry point returns a stntt_status; nothing throws and
+ * nothing exits (the reference exits on device errors,
+ * src/cuda/cu_util.cuh:15-28, and throws std::invalid_argument /
+ * std::out_of_range on bad sizes, src/ntt/ntt_cpu.cpp:48-50,
+ * src/cuda/cu_util.cuh:104-117).  stntt_last_error() gives the message of the
+ * last failure on the calling thread.
+ *
+ * Threading: a context is externally synchronised (one thread at a time);
+ * distinct contexts are independent.  Device entry points are asynchronous on
+ * the given stream (0 = the context's own stream).
+ */
+#ifndef STNTT_H
+#define STNTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum stntt_status {
+    STNTT_OK = 0,
+    STNTT_EINVAL = 1,  /* bad argument; the reference's std::invalid_argument */
+    STNTT_ERANGE = 2,  /* host buffer too small; the reference's std::out_of_range */
+    STNTT_ENOMEM = 3,  /* device allocation failed */
+    STNTT_EDEVICE = 4, /* HIP runtime error; the reference's CCErr exit */
+    STNTT_EAGAIN = 5,  /* serving: no slot / over the HBM budget right now */
+    STNTT_ENOTFOUND = 6 /* serving: unknown request id */
+} stntt_status;
+
+typedef enum stntt_field { STNTT_FIELD_U32 = 0, STNTT_FIELD_GOLDILOCKS = 1 } stntt_field;
+typedef enum stntt_direction { STNTT_FORWARD = 0, STNTT_INVERSE = 1 } stntt_direction;
+
+/* flags for the 32-bit field */
+#define STNTT_SIGNED_INPUT 1u /* inputs are int32 (negative values allowed, as in the reference) */
+
+typedef struct stntt_ctx stntt_ctx;
+typedef void *stntt_stream; /* a hipStream_t */
+
+/* ---- library / context ------------------------------------------------ */
+const char *stntt_version(void);
+const char *stntt_last_error(void);
+/* Context on one device: owns a stream, the resident twiddle plans and the
+   scratch pool.  Replaces the reference's process-global twiddle symbols
+   (src/cuda/ntt/implementations/common.cuh:8-9). */
+int stntt_ctx_create(int device, stntt_ctx **out);
+int stntt_ctx_destroy(stntt_ctx *ctx);
+stntt_stream stntt_ctx_stream(stntt_ctx *ctx);
+
+/* ---- parameters (host only) ------------------------------------------- */
+/* Smallest prime p = k n + 1 >= min_mod and root = g^((p-1)/n), g the
+   smallest generator.  Replaces findParams (src/ntt/ntt_util.cpp:10-14). */
+int stntt_find_params(uint64_t n, uint64_t min_mod, uint64_t *root, uint64_t *mod);
+
+/* ---- twiddle plans ------------------------------------------------------ */
+/* Builds (or finds) the resident twiddle plan and writes the logical table
+   T[i] = root^i mod p, i < n/2, natural order, to host memory -- the layout
+   of the reference's initTwiddles (src/cuda/ntt/implementations/common.cuh:
+   37-41).  The device keeps T factored as T[i] = HI[i >> b] * LO[i & (2^b-1)]
+   and the values written here are expanded from those factors on the GPU. */
+int stntt_twiddles_u32(stntt_ctx *ctx, uint64_t n, uint32_t root, uint32_t mod, int direction,
+                       uint32_t *host_out);
+int stntt_twiddles_goldilocks(stntt_ctx *ctx, uint32_t log2n, int direction, uint64_t *host_out);
+
+/* ---- transforms on device buffers (asynchronous) ----------------------- */
+/* In place on d_data[batches][2^log2n].  The GPU forms of nttStockham
+   (src/ntt/ntt_cpu.cpp:87-90) and of the kernel chain launched by
+   autoNtt<n> (src/cuda/ntt/ntt_kernel.cuh:169-184), for both directions. */
+int stntt_ntt_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t root,
+                  uint32_t mod, int direction, uint32_t flags, stntt_stream stream);
+int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                         stntt_stream stream);
+
+/* ---- transforms on host buffers (synchronous, autoNtt-shaped) ---------- */
+/* Copies host_vec[0 .. n*batches) to the device, transforms it, copies it
+   back over host_vec and returns the kernel-only time in *kernel_ms (HIP
+   events around the launches, as the reference's cudaEvent pair,
+   src/cuda/ntt/ntt_kernel.cuh:166-191).  host_len < n*batches is
+   STNTT_ERANGE (src/cuda/cu_util.cuh:115-117).  n must be a power of two
+   (STNTT_EINVAL otherwise).  The i32 form takes the reference's int storage:
+   any int input, reduced with the reference's modulo() semantics. */
+int stntt_auto_ntt_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                       uint32_t root, uint32_t mod, int direction, float *kernel_ms);
+int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_len, uint32_t n,
+                              uint32_t batches, int direction, float *kernel_ms);
+
+/* ---- device memory and stream plumbing --------------------------------- */
+int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr);
+int stntt_dev_free(stntt_ctx *ctx, void *d_ptr);
+int stntt_memcpy_htod(stntt_ctx *ctx, void *d_dst, const void *h_src, size_t bytes, stntt_stream stream);
+int stntt_memcpy_dtoh(stntt_ctx *ctx, void *h_dst, const void *d_src, size_t bytes, stntt_stream stream);
+int stntt_stream_sync(stntt_ctx *ctx, stntt_stream stream);
+/* Event pair on a stream: start/stop record on `stream`, elapsed waits for
+   stop and returns milliseconds. */
+int stntt_timer_start(stntt_ctx *ctx, stntt_stream stream);
+int stntt_timer_stop(stntt_ctx *ctx, stntt_stream stream, float *ms);
+
+/* ---- per-kernel timing ---------------------------------------------------
+   With profiling on, every kernel of a transform is bracketed by HIP events
+   on the stream it runs on (the reference brackets its launches with a
+   cudaEvent pair, src/cuda/ntt/ntt_kernel.cuh:166-191).  profile_read waits
+   for the last profiled transform and writes one duration per kernel. */
+int stntt_set_profiling(stntt_ctx *ctx, int on);
+int stntt_profile_read(stntt_ctx *ctx, float *kernel_ms, int capacity, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,58 @@
+// field_u32.hpp -- the reference's 32-bit prime field on gfx950 VALU.
+//
+// The reference keeps values in signed int and reduces with C '%'
+// (src/ntt/ntt_util.cpp:88-94, src/cuda/math.cuh:10-16); its products
+// w * (a - b) overflow int32 from p = 65537 on (src/ntt/ntt_cpu.cpp:74,
+// src/cuda/ntt/arithmetic.cuh:23).  Here p is any runtime prime below 2^31
+// (every findParams prime up to n = 2^27), values are canonical in [0, p),
+// and products are exact:
+//   * twiddles carry a Shoup companion wq = floor(w * 2^32 / p), so w * x
+//     costs one mul_hi and two mul_lo (Shoup, "NTL: A library for doing
+//     number theory", MulModPrecon);
+//   * a + b and a - b use the min(s, s - p) trick on unsigned words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stntt {
+
+struct U32Tw {
+    uint32_t w;   // omega^e mod p
+    uint32_t wq;  // floor(w * 2^32 / p)
+};
+
+__device__ __forceinline__ uint32_t u32_add(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t s = a + b;  // < 2p < 2^32
+    uint32_t t = s - p;
+    return s < t ? s : t;
+}
+
+__device__ __forceinline__ uint32_t u32_sub(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t d = a + p - b;  // in (0, 2p)
+    uint32_t t = d - p;
+    return d < t ? d : t;
+}
+
+// x * w mod p with w's Shoup companion; exact for any 32-bit x.
+__device__ __forceinline__ uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
+    uint32_t q = __umulhi(x, w.wq);
+    uint32_t r = x * w.w - q * p;  // in [0, 2p)
+    uint32_t t = r - p;
+    return r < t ? r : t;
+}
+
+// Reduce an arbitrary 32-bit input to [0, p).  bar = floor(2^32 / p);
+// x - floor(x * bar / 2^32) * p lies in [0, 2p).  For signed inputs a word
+// with the top bit set stands for x - 2^32, i.e. minus (2^32 mod p) = c32;
+// this reproduces the reference's modulo() on negative ints.
+__device__ __forceinline__ uint32_t u32_reduce_input(uint32_t x, uint32_t p, uint32_t bar, uint32_t c32,
+                                                     bool is_signed) {
+    uint32_t r = x - __umulhi(x, bar) * p;
+    uint32_t t = r - p;
+    r = r < t ? r : t;
+    if (is_signed && (int32_t)x < 0) r = u32_sub(r, c32, p);
+    return r;
+}
+
+}  // namespace stntt

@@ -1,0 +1,322 @@
+// ntt_pass.hpp -- one Stockham radix-R pass over LDS-resident tiles.
+//
+// The reference computes the DIF Stockham one radix-2 stage at a time
+// (src/ntt/ntt_cpu.cpp:66-79; on the GPU one launch and one HBM pass per
+// stage, src/cuda/ntt/implementations/st_ntt_global_radix2.cuh:95-117).
+// Stage i (J = 2^i), pair q < n/2, j = q mod J:
+//     y[2q - j] = a + b,   y[2q - j + J] = w^(q - j) (a - b),  a = x[q], b = x[q + n/2].
+// r consecutive radix-2 stages compose into one radix-r step:
+//     a_m = x[sJ + j + m n/r]                     (m < r)
+//     y[(r s + k) J + j] = w^(s J k) * sum_m a_m w_r^(m k)   (k < r),
+// with s < n/(rJ), j < J.  Because the arithmetic is exact, any grouping of
+// the stages gives the reference's output bit for bit.  A pass applies one
+// radix-R step (R up to 2^13) to a tile of C "columns" (consecutive t = sJ+j,
+// or C whole transforms when R = n), with the tile staged in LDS:
+//   1. global -> LDS, coalesced along whichever of (c, m) is contiguous;
+//   2. the R-point DFT of every column as radix-2^k sub-steps (k <= 4):
+//      each thread pulls r values of a column from LDS into registers, runs
+//      an r-point DIF butterfly network in registers, multiplies by the
+//      sub-step twiddle and writes back in Stockham order;
+//   3. LDS -> global with the pass twiddle w^(s J k), and on the last pass
+//      the inverse's n^-1 and canonicalisation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "field_gl.hpp"
+#include "field_u32.hpp"
+
+namespace stntt {
+
+// ---------------------------------------------------------------------------
+// compile-time helpers
+// ---------------------------------------------------------------------------
+template <int N, class F, int I = 0>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, F, I + 1>(static_cast<F &&>(f));
+    }
+}
+
+constexpr int bitrev_c(int x, int bits) {
+    int y = 0;
+    for (int i = 0; i < bits; ++i) y = (y << 1) | ((x >> i) & 1);
+    return y;
+}
+
+// Sub-step split of an R = 2^LOGR point DFT into ceil(LOGR/4) register
+// radices, as even as possible, larger ones first.
+constexpr int substeps_of(int logr) { return (logr + 3) / 4; }
+constexpr int substep_log(int logr, int i) {
+    int ns = substeps_of(logr);
+    return logr / ns + (i < logr % ns ? 1 : 0);
+}
+constexpr int substep_logj(int logr, int i) {
+    int s = 0;
+    for (int k = 0; k < i; ++k) s += substep_log(logr, k);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// field policies
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+    PASS_FIRST = 1u,   // reduce raw inputs (32-bit field)
+    PASS_LAST = 2u,    // canonicalise outputs, apply n^-1 for the inverse
+    PASS_SIGNED = 4u,  // 32-bit inputs are signed ints (reference int storage)
+    PASS_SCALE = 8u,   // multiply by n^-1 (inverse)
+};
+
+struct FieldGL {
+    using T = uint64_t;
+    using Tw = uint64_t;
+    struct Rt {
+        uint64_t ninv;
+    };
+    // runtime roots are not needed: omega_r^j is a power of two
+    struct Roots {};
+
+    static __device__ __forceinline__ T add(T a, T b, const Rt &) { return gl_add(a, b); }
+    static __device__ __forceinline__ T sub(T a, T b, const Rt &) { return gl_sub(a, b); }
+    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &) { return gl_mul(a, w); }
+    template <int LR, int DIR>
+    static __device__ __forceinline__ Roots load_roots(const Tw *, int) {
+        return {};
+    }
+    template <int LR, int DIR, int J>
+    static __device__ __forceinline__ T root_mul(T x, const Roots &, const Rt &) {
+        return gl_mul_pow2<gl_root_shift<LR, DIR>(J)>(x);
+    }
+    static __device__ __forceinline__ T load_in(T x, const Rt &, uint32_t) { return x; }
+    static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
+        if (flags & PASS_SCALE) x = gl_mul(x, rt.ninv);
+        return gl_canon(x);
+    }
+};
+
+struct FieldU32 {
+    using T = uint32_t;
+    using Tw = U32Tw;
+    struct Rt {
+        uint32_t p;    // modulus, < 2^31
+        uint32_t bar;  // floor(2^32 / p)
+        uint32_t c32;  // 2^32 mod p
+        U32Tw ninv;    // n^-1 with Shoup companion
+    };
+    struct Roots {
+        U32Tw w[8];  // omega_r^j, j < r/2
+    };
+
+    static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
+    static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32_sub(a, b, rt.p); }
+    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return u32_mul(a, w, rt.p); }
+    // omega_r^j = hi[j << (log_rmax - LR)] (hi[e] = omega_Rmax^e)
+    template <int LR, int DIR>
+    static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
+        Roots r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < (1 << LR) / 2) r.w[j] = hi[j << (log_rmax - LR)];
+        return r;
+    }
+    template <int LR, int DIR, int J>
+    static __device__ __forceinline__ T root_mul(T x, const Roots &r, const Rt &rt) {
+        if constexpr (J == 0) return x;
+        return u32_mul(x, r.w[J], rt.p);
+    }
+    static __device__ __forceinline__ T load_in(T x, const Rt &rt, uint32_t flags) {
+        if (flags & PASS_FIRST) return u32_reduce_input(x, rt.p, rt.bar, rt.c32, (flags & PASS_SIGNED) != 0);
+        return x;
+    }
+    static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
+        if (flags & PASS_SCALE) x = u32_mul(x, rt.ninv, rt.p);
+        return x;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// in-register r-point DFT: DIF radix-2 network, natural order in, output k
+// found at v[bitrev(k)].
+// ---------------------------------------------------------------------------
+template <class FP, int LR, int DIR>
+__device__ __forceinline__ void dft_regs(typename FP::T *v, const typename FP::Roots &roots,
+                                         const typename FP::Rt &rt) {
+    constexpr int r = 1 << LR;
+    static_for<LR>([&](auto st) {
+        constexpr int h = r >> (decltype(st)::value + 1);  // half block size
+        static_for<r / 2>([&](auto bi) {
+            constexpr int b = decltype(bi)::value;
+            constexpr int i = b % h;                     // position in half block
+            constexpr int base = (b / h) * 2 * h;
+            constexpr int jroot = i * (r / (2 * h));     // exponent of omega_r
+            auto x = v[base + i], y = v[base + i + h];
+            v[base + i] = FP::add(x, y, rt);
+            v[base + i + h] = FP::template root_mul<LR, DIR, jroot>(FP::sub(x, y, rt), roots, rt);
+        });
+    });
+}
+
+// ---------------------------------------------------------------------------
+// pass arguments
+// ---------------------------------------------------------------------------
+template <class FP>
+struct PassArgs {
+    const typename FP::T *in;
+    typename FP::T *out;
+    const typename FP::Tw *hi;  // hi[e] = omega^((n / Rmax) e), e < Rmax
+    const typename FP::Tw *lo;  // lo[e] = omega^e, e < n / Rmax
+    uint32_t log_n;
+    uint32_t log_j;     // J = product of the radices of earlier passes
+    uint32_t log_rmax;  // Rmax = min(n, 4096)
+    uint32_t batches;   // transforms in the buffer
+    uint32_t tiles_per_transform;
+    uint32_t flags;
+    typename FP::Rt rt;
+};
+
+// w^e for e < n through the two factor tables
+template <class FP>
+__device__ __forceinline__ typename FP::T apply_twiddle(typename FP::T x, uint64_t e, const PassArgs<FP> &a) {
+    uint32_t log_b = a.log_n - a.log_rmax;
+    uint64_t e_hi = e >> log_b, e_lo = e & ((1ull << log_b) - 1);
+    if (e_hi) x = FP::mul(x, a.hi[e_hi], a.rt);
+    if (e_lo) x = FP::mul(x, a.lo[e_lo], a.rt);
+    return x;
+}
+
+// One sub-step I of the in-LDS R-point DFT (then recurse to I + 1).
+template <class FP, int LOGR, int LOGC, int NT, int DIR, int I>
+__device__ __forceinline__ void lds_substeps(typename FP::T *lds, const PassArgs<FP> &a) {
+    using T = typename FP::T;
+    constexpr int R = 1 << LOGR, C = 1 << LOGC;
+    constexpr int LR = substep_log(LOGR, I);
+    constexpr int r = 1 << LR;
+    constexpr int LOGJ = substep_logj(LOGR, I);  // inner J'
+    constexpr bool LASTSUB = (LOGJ + LR == LOGR);
+    constexpr int NG = (R / r) * C;              // radix-r groups in the tile
+    constexpr int G = (NG + NT - 1) / NT;        // groups per thread
+    constexpr int STRIDE = R / r;
+
+    const auto roots = FP::template load_roots<LR, DIR>(a.hi, a.log_rmax);
+    T v[G][r];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        int q = threadIdx.x + gi * NT;
+        if (NG % NT == 0 || q < NG) {
+            int c = q & (C - 1), g = q >> LOGC;
+#pragma unroll
+            for (int m = 0; m < r; ++m) v[gi][m] = lds[(g + m * STRIDE) * C + c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        int q = threadIdx.x + gi * NT;
+        if (NG % NT == 0 || q < NG) {
+            int c = q & (C - 1), g = q >> LOGC;
+            int s = g >> LOGJ, j = g & ((1 << LOGJ) - 1);
+            dft_regs<FP, LR, DIR>(v[gi], roots, a.rt);
+#pragma unroll
+            for (int k = 0; k < r; ++k) {
+                T x = v[gi][bitrev_c(k, LR)];
+                if constexpr (!LASTSUB) {
+                    // omega_R^(s J' k) = hi[(s J' k) << (log_rmax - LOGR)]
+                    uint32_t e = (uint32_t)((s << LOGJ) * k);
+                    if (k != 0 && e != 0) x = FP::mul(x, a.hi[e << (a.log_rmax - LOGR)], a.rt);
+                }
+                lds[((r * s + k) * (1 << LOGJ) + j) * C + c] = x;
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (!LASTSUB) lds_substeps<FP, LOGR, LOGC, NT, DIR, I + 1>(lds, a);
+}
+
+// Multi-pass tile: C consecutive t = sJ + j of one transform.
+// Single-pass tile (R = n): C whole transforms of the batch.
+template <class FP, int LOGR, int LOGC, int NT, int DIR, bool SINGLE>
+__global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
+    using T = typename FP::T;
+    constexpr int R = 1 << LOGR, C = 1 << LOGC;
+    __shared__ T lds[R * C];
+
+    const uint64_t n = 1ull << a.log_n;
+    uint64_t in_base, out_base, cs_in, ms_in, cs_out, ks_out;
+    uint32_t ncols = C;
+    uint64_t t0 = 0;
+    bool j_is_one = a.log_j == 0;
+    if constexpr (SINGLE) {
+        uint64_t b0 = (uint64_t)blockIdx.x * C;
+        in_base = out_base = b0 << a.log_n;
+        cs_in = cs_out = n;
+        ms_in = ks_out = 1;
+        uint64_t left = a.batches - b0;
+        ncols = left < (uint64_t)C ? (uint32_t)left : (uint32_t)C;
+    } else {
+        uint64_t b = blockIdx.x / a.tiles_per_transform;
+        uint64_t ti = blockIdx.x % a.tiles_per_transform;
+        t0 = ti * C;
+        in_base = (b << a.log_n) + t0;
+        cs_in = 1;
+        ms_in = n >> LOGR;
+        if (j_is_one) {
+            out_base = (b << a.log_n) + t0 * R;
+            cs_out = R;
+            ks_out = 1;
+        } else {
+            uint64_t s = t0 >> a.log_j, j0 = t0 & ((1ull << a.log_j) - 1);
+            out_base = (b << a.log_n) + ((s * R) << a.log_j) + j0;
+            cs_out = 1;
+            ks_out = 1ull << a.log_j;
+        }
+    }
+
+    // 1. global -> LDS
+    const bool in_cfast = (cs_in == 1);
+    for (int idx = threadIdx.x; idx < R * C; idx += NT) {
+        int c, m;
+        if (in_cfast) {
+            c = idx & (C - 1);
+            m = idx >> LOGC;
+        } else {
+            m = idx & (R - 1);
+            c = idx >> LOGR;
+        }
+        T x = 0;
+        if ((uint32_t)c < ncols) x = FP::load_in(a.in[in_base + c * cs_in + m * ms_in], a.rt, a.flags);
+        lds[m * C + c] = x;
+    }
+    __syncthreads();
+
+    // 2. R-point DFT of every column
+    lds_substeps<FP, LOGR, LOGC, NT, DIR, 0>(lds, a);
+
+    // 3. LDS -> global with the pass twiddle w^(s J k)
+    const bool out_cfast = (cs_out == 1);
+    for (int idx = threadIdx.x; idx < R * C; idx += NT) {
+        int c, k;
+        if (out_cfast) {
+            c = idx & (C - 1);
+            k = idx >> LOGC;
+        } else {
+            k = idx & (R - 1);
+            c = idx >> LOGR;
+        }
+        if ((uint32_t)c >= ncols) continue;
+        T x = lds[k * C + c];
+        if constexpr (!SINGLE) {
+            // s J = t - j: with J = 1 it is t itself, otherwise the tile's s J
+            uint64_t sJ = j_is_one ? (t0 + c) : ((t0 >> a.log_j) << a.log_j);
+            uint64_t e = sJ * (uint64_t)k;
+            if (e) x = apply_twiddle<FP>(x, e, a);
+        }
+        if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
+        a.out[out_base + c * cs_out + k * ks_out] = x;
+    }
+}
+
+}  // namespace stntt

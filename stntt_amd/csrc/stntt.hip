@@ -1,0 +1,606 @@
+// stntt.hip -- C ABI, contexts, resident twiddle plans and pass scheduling
+// for the MI355X Stockham NTT.  Declarations and reference citations are in
+// include/stntt.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/stntt.h"
+#include "ntt_pass.hpp"
+#include "plan.hpp"
+
+namespace stntt {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(STNTT_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                                 \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host modular arithmetic (exact, 128-bit products)
+// ---------------------------------------------------------------------------
+using u128 = unsigned __int128;
+
+static uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    b %= m;
+    while (e) {
+        if (e & 1) r = (uint64_t)((u128)r * b % m);
+        b = (uint64_t)((u128)b * b % m);
+        e >>= 1;
+    }
+    return r;
+}
+
+// inverse by extended Euclid; 0 when a is not invertible
+static uint64_t invmod(uint64_t a, uint64_t m) {
+    __int128 t = 0, nt = 1, r = m, nr = a % m;
+    while (nr) {
+        __int128 q = r / nr, tmp;
+        tmp = t - q * nt; t = nt; nt = tmp;
+        tmp = r - q * nr; r = nr; nr = tmp;
+    }
+    if (r != 1) return 0;
+    if (t < 0) t += m;
+    return (uint64_t)t;
+}
+
+static inline U32Tw shoup(uint32_t w, uint32_t p) { return {w, (uint32_t)(((uint64_t)w << 32) / p)}; }
+
+static uint64_t gl_mul_host(uint64_t a, uint64_t b) { return (uint64_t)((u128)a * b % GL_P); }
+
+// ---------------------------------------------------------------------------
+// twiddle plans: resident factor tables
+//   hi[e] = w^((n/Rmax) e), e < Rmax;   lo[e] = w^e, e < n/Rmax
+// so that w^e = hi[e >> b] * lo[e & (2^b - 1)] with 2^b = n / Rmax.
+// ---------------------------------------------------------------------------
+struct PlanKey {
+    int field, dir;
+    uint32_t log_n;
+    uint64_t root, mod;
+    bool operator<(const PlanKey &o) const {
+        return std::tie(field, dir, log_n, root, mod) < std::tie(o.field, o.dir, o.log_n, o.root, o.mod);
+    }
+};
+
+struct Plan {
+    PlanKey key{};
+    void *d_tables = nullptr;  // hi then lo
+    size_t bytes = 0;
+    uint32_t log_rmax = 0;
+    FieldGL::Rt rt_gl{};
+    FieldU32::Rt rt_u32{};
+    const void *hi() const { return d_tables; }
+    const void *lo() const {
+        size_t esz = key.field == STNTT_FIELD_GOLDILOCKS ? sizeof(uint64_t) : sizeof(U32Tw);
+        return (const char *)d_tables + (esz << log_rmax);
+    }
+};
+
+}  // namespace stntt
+
+struct stntt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::map<stntt::PlanKey, std::unique_ptr<stntt::Plan>> plans;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void *staging = nullptr;  // device buffer of the host-buffer entry points
+    size_t staging_bytes = 0;
+    // per-pass timing: when on, events bracket every kernel of a transform
+    bool profile = false;
+    static constexpr int MAX_PROFILE_EVENTS = 9;
+    hipEvent_t prof_ev[MAX_PROFILE_EVENTS] = {};
+    int prof_count = 0;  // events recorded by the last profiled transform
+};
+
+namespace stntt {
+
+static int set_device(stntt_ctx *ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return STNTT_OK;
+}
+
+static int ensure_buffer(void **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return STNTT_OK;
+    if (*buf) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(*buf));
+        *buf = nullptr;
+        *cap = 0;
+    }
+    if (hipMalloc(buf, bytes) != hipSuccess) {
+        *buf = nullptr;
+        return fail(STNTT_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    *cap = bytes;
+    return STNTT_OK;
+}
+
+static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, uint64_t mod, int dir,
+                    const Plan **out) {
+    PlanKey key{field, dir, log_n, root, mod};
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) {
+        *out = it->second.get();
+        return STNTT_OK;
+    }
+    auto plan = std::make_unique<Plan>();
+    plan->key = key;
+    uint32_t log_rmax = log_n < MAX_LOGR ? log_n : MAX_LOGR;
+    plan->log_rmax = log_rmax;
+    uint64_t n = 1ull << log_n, rmax = 1ull << log_rmax, nlo = n >> log_rmax;
+    if (field == STNTT_FIELD_GOLDILOCKS) {
+        uint64_t w = log_n ? powmod(7, (GL_P - 1) >> log_n, GL_P) : 1;
+        if (dir == STNTT_INVERSE) w = invmod(w, GL_P);
+        std::vector<uint64_t> tab(rmax + nlo);
+        uint64_t step = powmod(w, nlo, GL_P), t = 1;
+        for (uint64_t e = 0; e < rmax; ++e, t = gl_mul_host(t, step)) tab[e] = t;
+        t = 1;
+        for (uint64_t e = 0; e < nlo; ++e, t = gl_mul_host(t, w)) tab[rmax + e] = t;
+        plan->rt_gl.ninv = invmod(n % GL_P, GL_P);
+        plan->bytes = tab.size() * sizeof(uint64_t);
+        HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
+        HIP_TRY(hipMemcpy(plan->d_tables, tab.data(), plan->bytes, hipMemcpyHostToDevice));
+    } else {
+        uint32_t p = (uint32_t)mod;
+        uint64_t w = root % p;
+        if (dir == STNTT_INVERSE) w = invmod(w, p);
+        std::vector<U32Tw> tab(rmax + nlo);
+        uint64_t step = powmod(w, nlo, p), t = 1 % p;
+        for (uint64_t e = 0; e < rmax; ++e, t = t * step % p) tab[e] = shoup((uint32_t)t, p);
+        t = 1 % p;
+        for (uint64_t e = 0; e < nlo; ++e, t = t * w % p) tab[rmax + e] = shoup((uint32_t)t, p);
+        plan->rt_u32.p = p;
+        plan->rt_u32.bar = (uint32_t)((1ull << 32) / p);
+        plan->rt_u32.c32 = (uint32_t)((1ull << 32) % p);
+        plan->rt_u32.ninv = shoup((uint32_t)invmod(n % p, p), p);
+        plan->bytes = tab.size() * sizeof(U32Tw);
+        HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
+        HIP_TRY(hipMemcpy(plan->d_tables, tab.data(), plan->bytes, hipMemcpyHostToDevice));
+    }
+    *out = plan.get();
+    ctx->plans.emplace(key, std::move(plan));
+    return STNTT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel dispatch: one template instance per (field, dir, LOGR, single)
+// ---------------------------------------------------------------------------
+template <class FP, int DIR, bool SINGLE, int LOGR>
+static void launch_one(const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+    constexpr int LOGC = pass_logc<FP>(LOGR, SINGLE);
+    hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, LOGC, PASS_THREADS, DIR, SINGLE>), dim3(grid),
+                       dim3(PASS_THREADS), 0, s, a);
+}
+
+template <class FP, int DIR, bool SINGLE, int BASE, int... L>
+static void launch_pass_impl(int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
+                             std::integer_sequence<int, L...>) {
+    ((logr == L + BASE ? (launch_one<FP, DIR, SINGLE, L + BASE>(a, grid, s), 0) : 0), ...);
+}
+
+template <class FP>
+static void launch_pass(int dir, bool single, int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+    using SeqS = std::make_integer_sequence<int, MAX_LOGR_SINGLE<FP>>;
+    using SeqM = std::make_integer_sequence<int, MAX_LOGR_MULTI - MIN_LOGR_MULTI + 1>;
+    if (dir == 0) {
+        if (single) launch_pass_impl<FP, 0, true, 1>(logr, a, grid, s, SeqS{});
+        else launch_pass_impl<FP, 0, false, MIN_LOGR_MULTI>(logr, a, grid, s, SeqM{});
+    } else {
+        if (single) launch_pass_impl<FP, 1, true, 1>(logr, a, grid, s, SeqS{});
+        else launch_pass_impl<FP, 1, false, MIN_LOGR_MULTI>(logr, a, grid, s, SeqM{});
+    }
+}
+
+template <class FP>
+__global__ void copy_kernel(const typename FP::T *in, typename FP::T *out, uint64_t count) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = in[i];
+}
+
+// Runs the pass schedule of `plan` over d_data[batches][n] in place.
+template <class FP>
+static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_data, uint64_t batches,
+                         uint32_t flags, hipStream_t s) {
+    using T = typename FP::T;
+    const uint32_t L = plan.key.log_n;
+    if (batches == 0) return STNTT_OK;
+    if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
+    Schedule sch = make_schedule<FP>(L);
+    const bool inv = plan.key.dir == STNTT_INVERSE;
+    T *scratch = nullptr;
+    const uint64_t total = batches << L;
+    if (sch.npasses > 1) {
+        int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, total * sizeof(T));
+        if (rc) return rc;
+        scratch = (T *)ctx->scratch;
+    }
+    PassArgs<FP> a{};
+    a.hi = (const typename FP::Tw *)plan.hi();
+    a.lo = (const typename FP::Tw *)plan.lo();
+    a.log_n = L;
+    a.log_rmax = plan.log_rmax;
+    a.batches = (uint32_t)batches;
+    if constexpr (std::is_same_v<FP, FieldGL>) a.rt = plan.rt_gl;
+    else a.rt = plan.rt_u32;
+
+    T *src = d_data;
+    uint32_t log_j = 0;
+    const bool prof = ctx->profile && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
+    if (prof) {
+        ctx->prof_count = 0;
+        HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
+    }
+    for (int p = 0; p < sch.npasses; ++p) {
+        const int logr = sch.logr[p];
+        const bool first = p == 0, last = p == sch.npasses - 1;
+        T *dst = sch.npasses == 1 ? d_data : (src == d_data ? scratch : d_data);
+        a.in = src;
+        a.out = dst;
+        a.log_j = log_j;
+        a.flags = (first ? (PASS_FIRST | (flags & STNTT_SIGNED_INPUT ? PASS_SIGNED : 0)) : 0) |
+                  (last ? (PASS_LAST | (inv ? PASS_SCALE : 0)) : 0);
+        uint32_t grid;
+        if (sch.single) {
+            uint64_t c = 1ull << pass_logc<FP>(logr, true);
+            grid = (uint32_t)((batches + c - 1) / c);
+        } else {
+            uint64_t c = 1ull << pass_logc<FP>(logr, false);
+            a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) / c);
+            grid = (uint32_t)(batches * a.tiles_per_transform);
+        }
+        launch_pass<FP>(plan.key.dir, sch.single, logr, a, grid, s);
+        HIP_TRY(hipGetLastError());
+        if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
+        src = dst;
+        log_j += logr;
+    }
+    if (src != d_data) {
+        uint64_t blocks = (total + 255) / 256;
+        hipLaunchKernelGGL(copy_kernel<FP>, dim3((uint32_t)blocks), dim3(256), 0, s, src, d_data, total);
+        HIP_TRY(hipGetLastError());
+        if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
+    }
+    return STNTT_OK;
+}
+
+template <class FP>
+__global__ void expand_twiddles_kernel(typename FP::T *out, const typename FP::Tw *hi, const typename FP::Tw *lo,
+                                       uint32_t log_b, uint64_t count, typename FP::Rt rt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    typename FP::T x;
+    if constexpr (std::is_same_v<FP, FieldGL>) {
+        x = gl_canon(gl_mul(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)]));
+    } else {
+        x = u32_mul(hi[i >> log_b].w, lo[i & ((1ull << log_b) - 1)], rt.p);
+    }
+    out[i] = x;
+}
+
+// ---------------------------------------------------------------------------
+// argument checks shared by the entry points
+// ---------------------------------------------------------------------------
+static int check_u32_params(uint32_t log2n, uint64_t root, uint64_t mod, int dir) {
+    if (log2n > 30) return fail(STNTT_EINVAL, "log2n %u too large", log2n);
+    if (mod < 2 || mod >= (1ull << 31))
+        return fail(STNTT_EINVAL, "mod %llu outside [2, 2^31)", (unsigned long long)mod);
+    if (dir != STNTT_FORWARD && dir != STNTT_INVERSE) return fail(STNTT_EINVAL, "bad direction %d", dir);
+    if (log2n >= 1) {
+        uint64_t half = powmod(root, 1ull << (log2n - 1), mod);
+        if (half != mod - 1)
+            return fail(STNTT_EINVAL, "root %llu is not of order n = 2^%u mod %llu (root^(n/2) != -1)",
+                        (unsigned long long)root, log2n, (unsigned long long)mod);
+        if (dir == STNTT_INVERSE && invmod((1ull << log2n) % mod, mod) == 0)
+            return fail(STNTT_EINVAL, "n is not invertible mod %llu", (unsigned long long)mod);
+    }
+    return STNTT_OK;
+}
+
+static int log2_exact(uint64_t n, uint32_t *out) {
+    if (n == 0 || (n & (n - 1))) return fail(STNTT_EINVAL, "size (%llu) must be a power of 2", (unsigned long long)n);
+    uint32_t l = 0;
+    while ((1ull << l) < n) ++l;
+    *out = l;
+    return STNTT_OK;
+}
+
+template <class T, class Run>
+static int auto_ntt_common(stntt_ctx *ctx, T *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                           float *kernel_ms, Run run) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    uint32_t L;
+    int rc = log2_exact(n, &L);
+    if (rc) return rc;
+    if (batches == 0) return fail(STNTT_EINVAL, "batches must be positive");
+    size_t count = (size_t)n * batches;
+    if (host_len < count || !host_vec)
+        return fail(STNTT_ERANGE, "host buffer (%zu elements) is smaller than device buffer (%zu)", host_len, count);
+    if ((rc = set_device(ctx))) return rc;
+    if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, count * sizeof(T)))) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->staging, host_vec, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    if ((rc = run(L, (T *)ctx->staging))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host_vec, ctx->staging, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = ms;
+    return STNTT_OK;
+}
+
+}  // namespace stntt
+
+using namespace stntt;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *stntt_version(void) { return "stntt-mi355x 0.1 (gfx950)"; }
+
+const char *stntt_last_error(void) { return g_err.c_str(); }
+
+int stntt_ctx_create(int device, stntt_ctx **out) {
+    if (!out) return fail(STNTT_EINVAL, "null out pointer");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(STNTT_EDEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(STNTT_EINVAL, "device %d out of range", device);
+    auto ctx = std::make_unique<stntt_ctx>();
+    ctx->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ctx->ev_start));
+    HIP_TRY(hipEventCreate(&ctx->ev_stop));
+    for (auto &e : ctx->prof_ev) HIP_TRY(hipEventCreate(&e));
+    *out = ctx.release();
+    return STNTT_OK;
+}
+
+int stntt_ctx_destroy(stntt_ctx *ctx) {
+    if (!ctx) return STNTT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->plans) (void)hipFree(kv.second->d_tables);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->staging) (void)hipFree(ctx->staging);
+    (void)hipEventDestroy(ctx->ev_start);
+    (void)hipEventDestroy(ctx->ev_stop);
+    for (auto &e : ctx->prof_ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return STNTT_OK;
+}
+
+stntt_stream stntt_ctx_stream(stntt_ctx *ctx) { return ctx ? (stntt_stream)ctx->stream : nullptr; }
+
+int stntt_find_params(uint64_t n, uint64_t min_mod, uint64_t *root, uint64_t *mod) {
+    if (!root || !mod) return fail(STNTT_EINVAL, "null output pointer");
+    if (n == 0) return fail(STNTT_EINVAL, "n must be positive");
+    // smallest prime p = k n + 1 >= min_mod (trial division)
+    uint64_t k = (min_mod > 1 ? (min_mod - 1 + n - 1) / n : 1);
+    if (k < 1) k = 1;
+    auto is_prime = [](uint64_t v) {
+        if (v < 2) return false;
+        for (uint64_t d = 2; (u128)d * d <= v; ++d)
+            if (v % d == 0) return false;
+        return true;
+    };
+    uint64_t p;
+    for (;; ++k) {
+        if (k > UINT64_MAX / n) return fail(STNTT_EINVAL, "search range exhausted");
+        p = k * n + 1;
+        if (is_prime(p)) break;
+    }
+    // distinct prime factors of p - 1, then the smallest generator
+    std::vector<uint64_t> f;
+    uint64_t m = p - 1;
+    for (uint64_t d = 2; (u128)d * d <= m; ++d)
+        if (m % d == 0) {
+            f.push_back(d);
+            while (m % d == 0) m /= d;
+        }
+    if (m > 1) f.push_back(m);
+    uint64_t g = 0;
+    for (uint64_t c = 1; c < p && !g; ++c) {
+        if (powmod(c, p - 1, p) != 1) continue;
+        bool ok = true;
+        for (uint64_t q : f)
+            if (powmod(c, (p - 1) / q, p) == 1) {
+                ok = false;
+                break;
+            }
+        if (ok) g = c;
+    }
+    if (!g) return fail(STNTT_EINVAL, "no generator mod %llu", (unsigned long long)p);
+    *mod = p;
+    *root = powmod(g, (p - 1) / n, p);
+    return STNTT_OK;
+}
+
+int stntt_ntt_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t root, uint32_t mod,
+                  int direction, uint32_t flags, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    int rc = check_u32_params(log2n, root, mod, direction);
+    if (rc) return rc;
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U32, log2n, root % mod, mod, direction, &plan))) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    return run_transform<FieldU32>(ctx, *plan, d_data, batches, flags, s);
+}
+
+int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                         stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (log2n > 32) return fail(STNTT_EINVAL, "log2n %u exceeds the 2-adicity 32 of P - 1", log2n);
+    if (direction != STNTT_FORWARD && direction != STNTT_INVERSE)
+        return fail(STNTT_EINVAL, "bad direction %d", direction);
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, direction, &plan))) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    return run_transform<FieldGL>(ctx, *plan, d_data, batches, 0, s);
+}
+
+int stntt_twiddles_u32(stntt_ctx *ctx, uint64_t n, uint32_t root, uint32_t mod, int direction, uint32_t *host_out) {
+    if (!ctx || !host_out) return fail(STNTT_EINVAL, "null argument");
+    uint32_t L;
+    int rc = log2_exact(n, &L);
+    if (rc) return rc;
+    if ((rc = check_u32_params(L, root, mod, direction))) return rc;
+    if (n < 2) return STNTT_OK;
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U32, L, root % mod, mod, direction, &plan))) return rc;
+    uint64_t count = n / 2;
+    if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, count * 4))) return rc;
+    hipLaunchKernelGGL(expand_twiddles_kernel<FieldU32>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint32_t *)ctx->staging, (const U32Tw *)plan->hi(), (const U32Tw *)plan->lo(),
+                       L - plan->log_rmax, count, plan->rt_u32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_out, ctx->staging, count * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return STNTT_OK;
+}
+
+int stntt_twiddles_goldilocks(stntt_ctx *ctx, uint32_t log2n, int direction, uint64_t *host_out) {
+    if (!ctx || !host_out) return fail(STNTT_EINVAL, "null argument");
+    if (log2n > 32) return fail(STNTT_EINVAL, "log2n %u too large", log2n);
+    if (log2n == 0) return STNTT_OK;
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, direction, &plan))) return rc;
+    uint64_t count = (1ull << log2n) / 2;
+    if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, count * 8))) return rc;
+    hipLaunchKernelGGL(expand_twiddles_kernel<FieldGL>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint64_t *)ctx->staging, (const uint64_t *)plan->hi(),
+                       (const uint64_t *)plan->lo(), log2n - plan->log_rmax, count, plan->rt_gl);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_out, ctx->staging, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return STNTT_OK;
+}
+
+int stntt_auto_ntt_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches, uint32_t root,
+                       uint32_t mod, int direction, float *kernel_ms) {
+    return auto_ntt_common<int32_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, int32_t *d) {
+        return stntt_ntt_u32(ctx, (uint32_t *)d, L, batches, root, mod, direction, STNTT_SIGNED_INPUT, nullptr);
+    });
+}
+
+int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                              int direction, float *kernel_ms) {
+    return auto_ntt_common<uint64_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, uint64_t *d) {
+        return stntt_ntt_goldilocks(ctx, d, L, batches, direction, nullptr);
+    });
+}
+
+int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr) {
+    if (!ctx || !d_ptr) return fail(STNTT_EINVAL, "null argument");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
+        *d_ptr = nullptr;
+        return fail(STNTT_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    return STNTT_OK;
+}
+
+int stntt_dev_free(stntt_ctx *ctx, void *d_ptr) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    HIP_TRY(hipFree(d_ptr));
+    return STNTT_OK;
+}
+
+int stntt_memcpy_htod(stntt_ctx *ctx, void *d_dst, const void *h_src, size_t bytes, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return STNTT_OK;
+}
+
+int stntt_memcpy_dtoh(stntt_ctx *ctx, void *h_dst, const void *d_src, size_t bytes, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return STNTT_OK;
+}
+
+int stntt_stream_sync(stntt_ctx *ctx, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(stream ? (hipStream_t)stream : ctx->stream));
+    return STNTT_OK;
+}
+
+int stntt_set_profiling(stntt_ctx *ctx, int on) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    ctx->profile = on != 0;
+    ctx->prof_count = 0;
+    return STNTT_OK;
+}
+
+int stntt_profile_read(stntt_ctx *ctx, float *kernel_ms, int capacity, int *count) {
+    if (!ctx || !count) return fail(STNTT_EINVAL, "null argument");
+    int k = ctx->prof_count > 0 ? ctx->prof_count - 1 : 0;
+    *count = k;
+    if (k == 0) return STNTT_OK;
+    HIP_TRY(hipEventSynchronize(ctx->prof_ev[k]));
+    for (int i = 0; i < k && i < capacity; ++i)
+        HIP_TRY(hipEventElapsedTime(&kernel_ms[i], ctx->prof_ev[i], ctx->prof_ev[i + 1]));
+    return STNTT_OK;
+}
+
+int stntt_timer_start(stntt_ctx *ctx, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    HIP_TRY(hipEventRecord(ctx->ev_start, stream ? (hipStream_t)stream : ctx->stream));
+    return STNTT_OK;
+}
+
+int stntt_timer_stop(stntt_ctx *ctx, stntt_stream stream, float *ms) {
+    if (!ctx || !ms) return fail(STNTT_EINVAL, "null argument");
+    HIP_TRY(hipEventRecord(ctx->ev_stop, stream ? (hipStream_t)stream : ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev_stop));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->ev_start, ctx->ev_stop));
+    return STNTT_OK;
+}
+
+}  // extern "C"
