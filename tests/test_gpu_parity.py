@@ -143,9 +143,14 @@ def test_signed_inputs_follow_reference_modulo(ctx):
         S.ntt_stockham(got, n, root, mod, batches=2, ctx=ctx)
         assert np.array_equal(got.astype(np.uint32), want), L
         if L <= 13 and O.ref_available():
-            r = x.copy()
+            # the reference's first stage adds raw ints (src/ntt/ntt_cpu.cpp:73), which wraps for
+            # full-range inputs; with |x| < 2^29 it is exact and must agree
+            small = (x >> np.int32(3)).astype(np.int32)
+            r = small.copy()
             assert O.ref().ref_ntt_stockham(r, n, root, mod, 2, -1) == 0
-            assert np.array_equal(got, r)
+            g2 = small.copy()
+            S.ntt_stockham(g2, n, root, mod, batches=2, ctx=ctx)
+            assert np.array_equal(g2, r), L
 
 
 def test_size_one_is_identity(ctx):
