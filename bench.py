@@ -47,6 +47,8 @@ def parse():
                     help="FETCH_SIZE/WRITE_SIZE per dispatch measured by rocprofv3 (optional)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of the same buffer (counter calibration)")
     return ap.parse_args()
 
 
@@ -99,6 +101,13 @@ def main() -> int:
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e9
+
+    if args.copy_calibration:
+        dst = ctx.alloc(n * 8)
+        for _ in range(3):
+            ctx.copy(dst, buf, n * 8)
+        ctx.sync()
+        dst.free()
 
     # per-kernel durations with HIP events on the library's stream
     ctx.set_profiling(True)

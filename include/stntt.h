@@ -102,6 +102,8 @@ int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr);
 int stntt_dev_free(stntt_ctx *ctx, void *d_ptr);
 int stntt_memcpy_htod(stntt_ctx *ctx, void *d_dst, const void *h_src, size_t bytes, stntt_stream stream);
 int stntt_memcpy_dtoh(stntt_ctx *ctx, void *h_dst, const void *d_src, size_t bytes, stntt_stream stream);
+/* device-to-device copy by a plain streaming kernel (asynchronous); bytes % 8 == 0 */
+int stntt_memcpy_dtod(stntt_ctx *ctx, void *d_dst, const void *d_src, size_t bytes, stntt_stream stream);
 int stntt_stream_sync(stntt_ctx *ctx, stntt_stream stream);
 /* Event pair on a stream: start/stop record on `stream`, elapsed waits for
    stop and returns milliseconds. */

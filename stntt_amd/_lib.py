@@ -51,6 +51,7 @@ SIGNATURES = [
     ("stntt_dev_free", _int, [_vp, _vp]),
     ("stntt_memcpy_htod", _int, [_vp, _vp, _vp, _sz, _vp]),
     ("stntt_memcpy_dtoh", _int, [_vp, _vp, _vp, _sz, _vp]),
+    ("stntt_memcpy_dtod", _int, [_vp, _vp, _vp, _sz, _vp]),
     ("stntt_stream_sync", _int, [_vp, _vp]),
     ("stntt_timer_start", _int, [_vp, _vp]),
     ("stntt_timer_stop", _int, [_vp, _vp, _c.POINTER(_c.c_float)]),

@@ -5,79 +5,116 @@
 // :19-30) but leaves mul and the butterfly empty (:32-35).  This file builds
 // the whole field from the special form of P:
 //     2^64 == 2^32 - 1 = EPS (mod P),   2^96 == -1 (mod P).
-// Values travel "weakly reduced": any 64-bit word is a valid representative
-// of its class mod P.  Only gl_canon() maps to [0, P), once, at the final
-// store.  Every routine is exact for all 64-bit inputs.
+//
+// Invariant: every value handed between routines is canonical, in [0, P).
+// Raw inputs are brought there once (gl_canon) when a transform loads them.
+// Keeping values canonical makes a + b and a - b one carry/borrow
+// correction each (no double wrap is possible), which is cheaper overall
+// than "any 64-bit word" representatives.
+//
+// On gfx950 the 32x32->64 multiply-add (v_mad_u64_u32), 64-bit add
+// (v_lshl_add_u64) and 64-bit compare all issue at full VALU rate, so cost is
+// simply instruction count; the sequences below are written for it.
+// Everything is __host__ __device__ so the CPU tests can check it exhaustively
+// on edge values.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#define GL_HD __host__ __device__ __forceinline__
 
 namespace stntt {
 
 constexpr uint64_t GL_P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t GL_EPS = 0x00000000FFFFFFFFull;
 
-// a + b (mod P).  A carry out of 2^64 is worth EPS; adding EPS can carry a
-// second time only when a or b was >= P, and then a second EPS settles it.
-__device__ __forceinline__ uint64_t gl_add(uint64_t a, uint64_t b) {
-    uint64_t s;
-    bool c1 = __builtin_add_overflow(a, b, &s);
-    uint64_t t;
-    bool c2 = __builtin_add_overflow(s, c1 ? GL_EPS : 0ull, &t);
-    return t + (c2 ? GL_EPS : 0ull);
+// any 64-bit word -> [0, P)
+GL_HD uint64_t gl_canon(uint64_t x) {
+    uint64_t t = x + GL_EPS;  // x - P (mod 2^64); wraps exactly when x >= P
+    return t < x ? t : x;
 }
 
-// a - b (mod P).  A borrow is worth -EPS; the second borrow case mirrors
-// gl_add.
-__device__ __forceinline__ uint64_t gl_sub(uint64_t a, uint64_t b) {
-    uint64_t d;
-    bool c1 = __builtin_sub_overflow(a, b, &d);
-    uint64_t t;
-    bool c2 = __builtin_sub_overflow(d, c1 ? GL_EPS : 0ull, &t);
-    return t - (c2 ? GL_EPS : 0ull);
+// a + b, a, b < P.  a + b < 2P: subtract P when the 64-bit sum carried or
+// reached P; s + EPS = s - P (mod 2^64) carries exactly in the second case.
+GL_HD uint64_t gl_add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    uint64_t t = s + GL_EPS;
+    return (s < a || t < s) ? t : s;
 }
 
-// lo + hi * 2^64 (mod P) with hi = h0 + h1 * 2^32:
-//     == lo - h1 + h0 * EPS,   and  h0 * EPS = (h0 << 32) - h0  < 2^64.
-__device__ __forceinline__ uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
-    uint64_t h0 = hi & 0xFFFFFFFFull, h1 = hi >> 32;
-    uint64_t t;
-    bool b = __builtin_sub_overflow(lo, h1, &t);
-    t -= b ? GL_EPS : 0ull;  // t was >= 2^64 - 2^32 + 1 > EPS: no second borrow
-    uint64_t u = (h0 << 32) - h0;
-    uint64_t r;
-    bool c = __builtin_add_overflow(t, u, &r);
-    return r + (c ? GL_EPS : 0ull);  // r < u here, so r + EPS < 2^64
+// a - b, a, b < P.  A borrow means a - b + 2^64; adding P is subtracting
+// EPS, which cannot borrow again (a - b + 2^64 > 2^32).
+GL_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
+    uint64_t d = a - b;
+    return a < b ? d - GL_EPS : d;
 }
 
-__device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
-    return gl_reduce128(a * b, __umul64hi(a, b));
+// lo + hi * 2^64 (mod P) for any lo, hi; result is any 64-bit word
+// (callers canonicalise).  hi = h0 + h1 2^32:  lo + h0 (2^32 - 1) - h1.
+//   r = h0 * EPS + lo         one v_mad_u64_u32; a carry is worth EPS and
+//                             the corrected sum stays below 2^64
+//   r - h1                    a borrow is worth -EPS; r - h1 + 2^64 > 2^32,
+//                             so no second borrow
+GL_HD uint64_t gl_reduce128_w(uint64_t lo, uint64_t hi) {
+    uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
+    uint64_t r = h0 * GL_EPS + lo;
+    if (r < lo) r += GL_EPS;
+    uint64_t t = r - h1;
+    if (r < h1) t -= GL_EPS;
+    return t;
 }
 
-// x * 2^S (mod P) for a compile-time S; 2^192 == 1 and 2^96 == -1, so only
-// S mod 96 shifts and S mod 192 >= 96 negates.
+GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) { return gl_canon(gl_reduce128_w(lo, hi)); }
+
+// 64 x 64 -> 128 from four 32 x 32 -> 64 multiply-adds
+GL_HD void gl_mul_wide(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+    uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+    uint64_t p00 = a0 * b0;
+    uint64_t p01 = a0 * b1 + (p00 >> 32);
+    uint64_t p10 = a1 * b0 + (uint32_t)p01;
+    hi = a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    lo = (p10 << 32) | (uint32_t)p00;
+}
+
+// a * b for any 64-bit a, b; result in [0, P)
+GL_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
+    uint64_t lo, hi;
+    gl_mul_wide(a, b, lo, hi);
+    return gl_reduce128(lo, hi);
+}
+
+// x * 2^s (mod P) for 0 < s <= 32 and any x; result is any 64-bit word.
+// x << s = lo + h 2^64 with h < 2^s, so lo + h EPS by one multiply-add and
+// one carry correction (the corrected value stays below 2^64).
+GL_HD uint64_t gl_shl_small_w(uint64_t x, int s) {
+    uint64_t lo = x << s;
+    uint64_t h = x >> (64 - s);
+    uint64_t r = h * GL_EPS + lo;
+    if (r < lo) r += GL_EPS;
+    return r;
+}
+
+// x * 2^s (mod P) for 0 < s < 64, any x; result is any 64-bit word
+GL_HD uint64_t gl_shl_w(uint64_t x, int s) {
+    if (s <= 32) return gl_shl_small_w(x, s);
+    return gl_reduce128_w(x << s, x >> (64 - s));
+}
+
+// x * 2^S (mod P) for a compile-time S and any x; result in [0, P).
+// 2^192 == 1 and 2^96 == -1.
 template <int S>
-__device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x) {
+GL_HD uint64_t gl_mul_pow2(uint64_t x) {
     constexpr int s192 = ((S % 192) + 192) % 192;
     constexpr bool neg = s192 >= 96;
     constexpr int s = s192 % 96;
     uint64_t r;
-    if constexpr (s == 0) {
-        r = x;
-    } else if constexpr (s < 64) {
-        r = gl_reduce128(x << s, x >> (64 - s));
-    } else {
-        // 2^s = 2^(s-32) * 2^32 with both shifts below 64
-        uint64_t y = gl_reduce128(x << (s - 32), x >> (96 - s));
-        r = gl_reduce128(y << 32, y >> 32);
-    }
-    if constexpr (neg) return gl_sub(0ull, r);
+    if constexpr (s == 0) r = gl_canon(x);
+    else if constexpr (s < 64) r = gl_canon(gl_shl_w(x, s));
+    else r = gl_canon(gl_shl_small_w(gl_shl_w(x, s - 32), 32));
+    if constexpr (neg) return gl_sub(0, r);
     return r;
 }
-
-// canonical representative in [0, P)
-__device__ __forceinline__ uint64_t gl_canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
 
 // omega_64 = 7^((P-1)/64) = 2^39 (mod P), so every root of unity of order
 // r | 64 is a power of two: omega_r^j = 2^(j * 39 * 64 / r).  Inverse roots
@@ -86,6 +123,17 @@ template <int LOGR, int DIR>
 constexpr int gl_root_shift(int j) {
     int e = (j * 39 * (64 >> LOGR)) % 192;
     return DIR == 0 ? e : (192 - e) % 192;
+}
+
+// (a - b) * 2^S for a, b < P with the sign of 2^S (2^96 == -1) folded
+// into the subtraction: (a - b)(-2^s) = (b - a) 2^s.  Result in [0, P).
+template <int S>
+GL_HD uint64_t gl_sub_mul_pow2(uint64_t a, uint64_t b) {
+    constexpr int s192 = ((S % 192) + 192) % 192;
+    if constexpr (s192 == 0) return gl_sub(a, b);
+    else if constexpr (s192 == 96) return gl_sub(b, a);
+    else if constexpr (s192 > 96) return gl_mul_pow2<s192 - 96>(gl_sub(b, a));
+    else return gl_mul_pow2<s192>(gl_sub(a, b));
 }
 
 }  // namespace stntt

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define U32_HD __host__ __device__ __forceinline__
+
 namespace stntt {
 
 struct U32Tw {
@@ -22,21 +24,21 @@ struct U32Tw {
     uint32_t wq;  // floor(w * 2^32 / p)
 };
 
-__device__ __forceinline__ uint32_t u32_add(uint32_t a, uint32_t b, uint32_t p) {
+U32_HD uint32_t u32_add(uint32_t a, uint32_t b, uint32_t p) {
     uint32_t s = a + b;  // < 2p < 2^32
     uint32_t t = s - p;
     return s < t ? s : t;
 }
 
-__device__ __forceinline__ uint32_t u32_sub(uint32_t a, uint32_t b, uint32_t p) {
+U32_HD uint32_t u32_sub(uint32_t a, uint32_t b, uint32_t p) {
     uint32_t d = a + p - b;  // in (0, 2p)
     uint32_t t = d - p;
     return d < t ? d : t;
 }
 
 // x * w mod p with w's Shoup companion; exact for any 32-bit x.
-__device__ __forceinline__ uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
-    uint32_t q = __umulhi(x, w.wq);
+U32_HD uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
+    uint32_t q = (uint32_t)(((uint64_t)x * w.wq) >> 32);
     uint32_t r = x * w.w - q * p;  // in [0, 2p)
     uint32_t t = r - p;
     return r < t ? r : t;
@@ -46,9 +48,9 @@ __device__ __forceinline__ uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
 // x - floor(x * bar / 2^32) * p lies in [0, 2p).  For signed inputs a word
 // with the top bit set stands for x - 2^32, i.e. minus (2^32 mod p) = c32;
 // this reproduces the reference's modulo() on negative ints.
-__device__ __forceinline__ uint32_t u32_reduce_input(uint32_t x, uint32_t p, uint32_t bar, uint32_t c32,
+U32_HD uint32_t u32_reduce_input(uint32_t x, uint32_t p, uint32_t bar, uint32_t c32,
                                                      bool is_signed) {
-    uint32_t r = x - __umulhi(x, bar) * p;
+    uint32_t r = x - (uint32_t)(((uint64_t)x * bar) >> 32) * p;
     uint32_t t = r - p;
     r = r < t ? r : t;
     if (is_signed && (int32_t)x < 0) r = u32_sub(r, c32, p);

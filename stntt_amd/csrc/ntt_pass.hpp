@@ -87,15 +87,21 @@ struct FieldGL {
     static __device__ __forceinline__ Roots load_roots(const Tw *, int) {
         return {};
     }
+    // (x - y) * omega_r^J: a shift by a power of two with the sign folded
+    // into the subtraction
     template <int LR, int DIR, int J>
-    static __device__ __forceinline__ T root_mul(T x, const Roots &, const Rt &) {
-        return gl_mul_pow2<gl_root_shift<LR, DIR>(J)>(x);
+    static __device__ __forceinline__ T sub_root_mul(T x, T y, const Roots &, const Rt &) {
+        return gl_sub_mul_pow2<gl_root_shift<LR, DIR>(J)>(x, y);
     }
-    static __device__ __forceinline__ T load_in(T x, const Rt &, uint32_t) { return x; }
+    // raw words enter the canonical range once, on the first pass
+    static __device__ __forceinline__ T load_in(T x, const Rt &, uint32_t flags) {
+        return (flags & PASS_FIRST) ? gl_canon(x) : x;
+    }
     static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
-        if (flags & PASS_SCALE) x = gl_mul(x, rt.ninv);
-        return gl_canon(x);
+        return (flags & PASS_SCALE) ? gl_mul(x, rt.ninv) : x;
     }
+    // twiddle hi * lo from the two factor tables
+    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &) { return gl_mul(hi, lo); }
 };
 
 struct FieldU32 {
@@ -108,7 +114,7 @@ struct FieldU32 {
         U32Tw ninv;    // n^-1 with Shoup companion
     };
     struct Roots {
-        U32Tw w[8];  // omega_r^j, j < r/2
+        U32Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
     };
 
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
@@ -119,14 +125,13 @@ struct FieldU32 {
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
         Roots r;
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j < (1 << LR) / 2) r.w[j] = hi[j << (log_rmax - LR)];
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = hi[j << (log_rmax - LR)];
         return r;
     }
     template <int LR, int DIR, int J>
-    static __device__ __forceinline__ T root_mul(T x, const Roots &r, const Rt &rt) {
-        if constexpr (J == 0) return x;
-        return u32_mul(x, r.w[J], rt.p);
+    static __device__ __forceinline__ T sub_root_mul(T x, T y, const Roots &r, const Rt &rt) {
+        if constexpr (J == 0) return u32_sub(x, y, rt.p);
+        return u32_mul(u32_sub(x, y, rt.p), r.w[J], rt.p);
     }
     static __device__ __forceinline__ T load_in(T x, const Rt &rt, uint32_t flags) {
         if (flags & PASS_FIRST) return u32_reduce_input(x, rt.p, rt.bar, rt.c32, (flags & PASS_SIGNED) != 0);
@@ -135,6 +140,11 @@ struct FieldU32 {
     static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
         if (flags & PASS_SCALE) x = u32_mul(x, rt.ninv, rt.p);
         return x;
+    }
+    // twiddle hi * lo with a fresh Shoup companion (one division per twiddle)
+    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) {
+        uint32_t w = u32_mul(hi.w, lo, rt.p);
+        return {w, (uint32_t)(((uint64_t)w << 32) / rt.p)};
     }
 };
 
@@ -155,7 +165,7 @@ __device__ __forceinline__ void dft_regs(typename FP::T *v, const typename FP::R
             constexpr int jroot = i * (r / (2 * h));     // exponent of omega_r
             auto x = v[base + i], y = v[base + i + h];
             v[base + i] = FP::add(x, y, rt);
-            v[base + i + h] = FP::template root_mul<LR, DIR, jroot>(FP::sub(x, y, rt), roots, rt);
+            v[base + i + h] = FP::template sub_root_mul<LR, DIR, jroot>(x, y, roots, rt);
         });
     });
 }

@@ -2,21 +2,23 @@
 //
 // The reference runs one global-memory launch per radix-2 stage until the
 // remaining stages fit one block (src/cuda/ntt/implementations/
-// st_ntt_global_radix2.cuh:98-116): 14 HBM passes at n = 2^24.  Here each
-// pass consumes up to 12-14 stages from a tile held in LDS, so n = 2^24 takes
-// 2 passes and n = 2^26 takes 3.
+// st_ntt_global_radix2.cuh:98-116): 14 HBM passes at n = 2^24.  Here a
+// small transform is one LDS pass, and a large one is 2-3 tile passes of up
+// to 9 stages each with whole 128-byte rows moved per access: n = 2^24 and
+// n = 2^26 take 3 passes.
 #pragma once
 
 #include <type_traits>
 
 #include "ntt_pass.hpp"
+#include "ntt_tile.hpp"
 
 namespace stntt {
 
 constexpr int PASS_THREADS = 256;
 constexpr int MAX_LOGR = 14;        // largest pass radix; also the size of the hi twiddle factor
-constexpr int MIN_LOGR_MULTI = 6;   // smallest radix a multi-pass schedule uses
-constexpr int MAX_LOGR_MULTI = 12;  // largest radix a multi-pass schedule uses
+constexpr int MIN_LOGR_TILE = 4;    // smallest radix of a tile pass
+constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile)
 
 // log2 of the tile size in elements: 64 KiB of LDS per workgroup
 template <class FP>
@@ -43,17 +45,27 @@ struct Schedule {
     int logr[4] = {0, 0, 0, 0};
 };
 
+// smallest L a tile schedule covers: every pass needs n / R >= C columns and
+// every later pass J >= C
 template <class FP>
-inline Schedule make_schedule(int L) {
+constexpr int min_tile_log() {
+    return 2 * tile_logc<FP>();
+}
+
+// Single LDS pass when the transform is small or the batch alone fills the
+// GPU; otherwise 2-3 tile passes of at most 2^9 points each (2^24: 8+8+8).
+template <class FP>
+inline Schedule make_schedule(int L, uint64_t batches) {
     Schedule s;
-    if (L <= MAX_LOGR_SINGLE<FP>) {
+    const bool fits_one = L <= MAX_LOGR_SINGLE<FP>;
+    if (L < min_tile_log<FP>() || (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
         s.single = true;
         s.npasses = 1;
         s.logr[0] = L;
         return s;
     }
     s.single = false;
-    int np = (L + MAX_LOGR_MULTI - 1) / MAX_LOGR_MULTI;
+    int np = (L + MAX_LOGR_TILE - 1) / MAX_LOGR_TILE;
     if (np < 2) np = 2;
     s.npasses = np;
     for (int i = 0; i < np; ++i) s.logr[i] = L / np + (i < L % np ? 1 : 0);

@@ -194,29 +194,43 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
 // ---------------------------------------------------------------------------
 // kernel dispatch: one template instance per (field, dir, LOGR, single)
 // ---------------------------------------------------------------------------
-template <class FP, int DIR, bool SINGLE, int LOGR>
-static void launch_one(const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
-    constexpr int LOGC = pass_logc<FP>(LOGR, SINGLE);
-    hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, LOGC, PASS_THREADS, DIR, SINGLE>), dim3(grid),
-                       dim3(PASS_THREADS), 0, s, a);
+template <class FP, int DIR, int LOGR>
+static void launch_single(const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+    constexpr int LOGC = pass_logc<FP>(LOGR, true);
+    hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, LOGC, PASS_THREADS, DIR, true>), dim3(grid), dim3(PASS_THREADS), 0,
+                       s, a);
 }
 
-template <class FP, int DIR, bool SINGLE, int BASE, int... L>
-static void launch_pass_impl(int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
+template <class FP, int DIR, int LOGR>
+static void launch_tile(bool first, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+    constexpr int NT = tile_threads<FP, LOGR>();
+    if (first) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true>), dim3(grid), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false>), dim3(grid), dim3(NT), 0, s, a);
+}
+
+template <class FP, int DIR, int... L>
+static void launch_single_impl(int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
+                               std::integer_sequence<int, L...>) {
+    ((logr == L + 1 ? (launch_single<FP, DIR, L + 1>(a, grid, s), 0) : 0), ...);
+}
+
+template <class FP, int DIR, int... L>
+static void launch_tile_impl(int logr, bool first, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
                              std::integer_sequence<int, L...>) {
-    ((logr == L + BASE ? (launch_one<FP, DIR, SINGLE, L + BASE>(a, grid, s), 0) : 0), ...);
+    ((logr == L + MIN_LOGR_TILE ? (launch_tile<FP, DIR, L + MIN_LOGR_TILE>(first, a, grid, s), 0) : 0), ...);
 }
 
 template <class FP>
-static void launch_pass(int dir, bool single, int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+static void launch_pass(int dir, bool single, bool first, int logr, const PassArgs<FP> &a, uint32_t grid,
+                        hipStream_t s) {
     using SeqS = std::make_integer_sequence<int, MAX_LOGR_SINGLE<FP>>;
-    using SeqM = std::make_integer_sequence<int, MAX_LOGR_MULTI - MIN_LOGR_MULTI + 1>;
+    using SeqT = std::make_integer_sequence<int, MAX_LOGR_TILE - MIN_LOGR_TILE + 1>;
     if (dir == 0) {
-        if (single) launch_pass_impl<FP, 0, true, 1>(logr, a, grid, s, SeqS{});
-        else launch_pass_impl<FP, 0, false, MIN_LOGR_MULTI>(logr, a, grid, s, SeqM{});
+        if (single) launch_single_impl<FP, 0>(logr, a, grid, s, SeqS{});
+        else launch_tile_impl<FP, 0>(logr, first, a, grid, s, SeqT{});
     } else {
-        if (single) launch_pass_impl<FP, 1, true, 1>(logr, a, grid, s, SeqS{});
-        else launch_pass_impl<FP, 1, false, MIN_LOGR_MULTI>(logr, a, grid, s, SeqM{});
+        if (single) launch_single_impl<FP, 1>(logr, a, grid, s, SeqS{});
+        else launch_tile_impl<FP, 1>(logr, first, a, grid, s, SeqT{});
     }
 }
 
@@ -234,14 +248,18 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     const uint32_t L = plan.key.log_n;
     if (batches == 0) return STNTT_OK;
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
-    Schedule sch = make_schedule<FP>(L);
+    Schedule sch = make_schedule<FP>(L, batches);
     const bool inv = plan.key.dir == STNTT_INVERSE;
-    T *scratch = nullptr;
+    // intermediates alternate between two scratch halves so that the last
+    // pass always lands in the caller's buffer (no copy-back)
+    T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
-        int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, total * sizeof(T));
+        const int halves = sch.npasses > 2 ? 2 : 1;
+        int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
         if (rc) return rc;
-        scratch = (T *)ctx->scratch;
+        scratch[0] = (T *)ctx->scratch;
+        scratch[1] = halves > 1 ? scratch[0] + total : nullptr;
     }
     PassArgs<FP> a{};
     a.hi = (const typename FP::Tw *)plan.hi();
@@ -262,7 +280,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     for (int p = 0; p < sch.npasses; ++p) {
         const int logr = sch.logr[p];
         const bool first = p == 0, last = p == sch.npasses - 1;
-        T *dst = sch.npasses == 1 ? d_data : (src == d_data ? scratch : d_data);
+        T *dst = last ? d_data : scratch[p & 1];
         a.in = src;
         a.out = dst;
         a.log_j = log_j;
@@ -273,21 +291,14 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             uint64_t c = 1ull << pass_logc<FP>(logr, true);
             grid = (uint32_t)((batches + c - 1) / c);
         } else {
-            uint64_t c = 1ull << pass_logc<FP>(logr, false);
-            a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) / c);
+            a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> tile_logc<FP>());
             grid = (uint32_t)(batches * a.tiles_per_transform);
         }
-        launch_pass<FP>(plan.key.dir, sch.single, logr, a, grid, s);
+        launch_pass<FP>(plan.key.dir, sch.single, first, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
         src = dst;
         log_j += logr;
-    }
-    if (src != d_data) {
-        uint64_t blocks = (total + 255) / 256;
-        hipLaunchKernelGGL(copy_kernel<FP>, dim3((uint32_t)blocks), dim3(256), 0, s, src, d_data, total);
-        HIP_TRY(hipGetLastError());
-        if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
     }
     return STNTT_OK;
 }
@@ -562,6 +573,18 @@ int stntt_memcpy_dtoh(stntt_ctx *ctx, void *h_dst, const void *d_src, size_t byt
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return STNTT_OK;
+}
+
+int stntt_memcpy_dtod(stntt_ctx *ctx, void *d_dst, const void *d_src, size_t bytes, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (bytes % 8) return fail(STNTT_EINVAL, "byte count %zu is not a multiple of 8", bytes);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    uint64_t count = bytes / 8;
+    if (count == 0) return STNTT_OK;
+    hipLaunchKernelGGL(copy_kernel<FieldGL>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, s,
+                       (const uint64_t *)d_src, (uint64_t *)d_dst, count);
+    HIP_TRY(hipGetLastError());
     return STNTT_OK;
 }
 

@@ -78,6 +78,9 @@ class Context:
         buf.copy_from(arr)
         return buf
 
+    def copy(self, dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int, stream: int | None = None) -> None:
+        check(lib().stntt_memcpy_dtod(self.handle, dst.ptr, src.ptr, nbytes, stream))
+
     def sync(self, stream: int | None = None) -> None:
         check(lib().stntt_stream_sync(self.handle, stream))
 

@@ -1,0 +1,101 @@
+// field_test.cpp -- host-side check of the device field arithmetic
+// (stntt_amd/csrc/field_gl.hpp, field_u32.hpp) against 128-bit integer
+// arithmetic on edge values and random values.  Runs on the CPU.
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../stntt_amd/csrc/field_gl.hpp"
+#include "../../stntt_amd/csrc/field_u32.hpp"
+
+using namespace stntt;
+typedef unsigned __int128 u128;
+
+static int fails = 0;
+#define CHECK(cond, ...) do { if (!(cond)) { if (fails < 20) { printf(__VA_ARGS__); printf("\n"); } ++fails; } } while (0)
+
+static uint64_t ref_mul(uint64_t a, uint64_t b) { return (uint64_t)((u128)a * b % GL_P); }
+static uint64_t ref_pow2(uint64_t x, int s) {
+    u128 r = x % GL_P;
+    for (int i = 0; i < s; ++i) r = (r * 2) % GL_P;
+    return (uint64_t)r;
+}
+
+template <int S>
+static void check_pow2(const std::vector<uint64_t> &vals) {
+    for (uint64_t x : vals) {
+        uint64_t got = gl_mul_pow2<S>(x), want = ref_pow2(x, S);
+        CHECK(got == want, "mul_pow2<%d>(%llx) = %llx want %llx", S, (unsigned long long)x, (unsigned long long)got,
+              (unsigned long long)want);
+    }
+}
+
+template <int... S>
+static void check_all_pow2(const std::vector<uint64_t> &vals, std::integer_sequence<int, S...>) {
+    (check_pow2<S>(vals), ...);
+}
+
+int main() {
+    std::vector<uint64_t> any = {0, 1, 2, 3, GL_EPS - 1, GL_EPS, GL_EPS + 1, 1ull << 32, (1ull << 32) + 1,
+                                 GL_P - 2, GL_P - 1, GL_P, GL_P + 1, ~0ull, ~0ull - 1, 1ull << 63, (1ull << 63) - 1,
+                                 0xFFFFFFFF00000000ull, 0x00000000FFFFFFFEull, 0x8000000080000000ull};
+    std::mt19937_64 rng(12345);
+    for (int i = 0; i < 3000; ++i) any.push_back(rng());
+    for (int i = 0; i < 500; ++i) any.push_back(GL_P - 1 - (rng() & 0xFFFFF));
+    for (int i = 0; i < 500; ++i) any.push_back(~0ull - (rng() & 0xFFFFFFFF));
+    std::vector<uint64_t> canon;
+    for (uint64_t x : any) canon.push_back(x % GL_P);
+
+    for (uint64_t x : any) CHECK(gl_canon(x) == x % GL_P, "canon %llx", (unsigned long long)x);
+    for (size_t i = 0; i < canon.size(); ++i)
+        for (size_t j = 0; j < canon.size(); j += 37) {
+            uint64_t a = canon[i], b = canon[j];
+            CHECK(gl_add(a, b) == (uint64_t)(((u128)a + b) % GL_P), "add %llx %llx", (unsigned long long)a,
+                  (unsigned long long)b);
+            CHECK(gl_sub(a, b) == (uint64_t)(((u128)a + GL_P - b) % GL_P), "sub %llx %llx", (unsigned long long)a,
+                  (unsigned long long)b);
+        }
+    for (size_t i = 0; i < any.size(); ++i)
+        for (size_t j = 0; j < any.size(); j += 41) {
+            uint64_t a = any[i], b = any[j];
+            CHECK(gl_mul(a, b) == ref_mul(a, b), "mul %llx %llx", (unsigned long long)a, (unsigned long long)b);
+            uint64_t lo = a, hi = b;
+            uint64_t want = (uint64_t)((((u128)hi << 64) | lo) % GL_P);
+            CHECK(gl_reduce128(lo, hi) == want, "reduce128 %llx %llx", (unsigned long long)lo, (unsigned long long)hi);
+        }
+    check_all_pow2(any, std::make_integer_sequence<int, 192>{});
+    // sign folding: (a - b) * 2^S
+    for (size_t i = 0; i < canon.size(); i += 7) {
+        uint64_t a = canon[i], b = canon[(i * 13 + 5) % canon.size()];
+        uint64_t d = (uint64_t)(((u128)a + GL_P - b) % GL_P);
+        CHECK(gl_sub_mul_pow2<120>(a, b) == ref_pow2(d, 120), "submul120");
+        CHECK(gl_sub_mul_pow2<36>(a, b) == ref_pow2(d, 36), "submul36");
+        CHECK(gl_sub_mul_pow2<96>(a, b) == ref_pow2(d, 96), "submul96");
+        CHECK(gl_sub_mul_pow2<0>(a, b) == d, "submul0");
+    }
+    // 32-bit field: Shoup product, add, sub, input reduction, for the
+    // findParams primes and the largest prime below 2^31
+    for (uint32_t p : {11u, 12289u, 65537u, 7340033u, 2013265921u, 2147483629u}) {
+        uint32_t bar = (uint32_t)((1ull << 32) / p), c32 = (uint32_t)((1ull << 32) % p);
+        std::vector<uint32_t> xs = {0, 1, p - 1, p / 2, 0xFFFFFFFFu, 0x80000000u, 0x7FFFFFFFu};
+        for (int i = 0; i < 400; ++i) xs.push_back((uint32_t)rng());
+        for (uint32_t x : xs) {
+            uint32_t canon_x = x % p;
+            CHECK(u32_reduce_input(x, p, bar, c32, false) == canon_x, "u32 reduce %u mod %u", x, p);
+            int64_t sx = (int32_t)x;
+            uint32_t want_s = (uint32_t)(((sx % (int64_t)p) + p) % p);
+            CHECK(u32_reduce_input(x, p, bar, c32, true) == want_s, "u32 signed reduce %u mod %u", x, p);
+            for (uint32_t y : xs) {
+                uint32_t a = x % p, b = y % p;
+                CHECK(u32_add(a, b, p) == (uint32_t)(((uint64_t)a + b) % p), "u32 add");
+                CHECK(u32_sub(a, b, p) == (uint32_t)(((uint64_t)a + p - b) % p), "u32 sub");
+                U32Tw w{b, (uint32_t)(((uint64_t)b << 32) / p)};
+                CHECK(u32_mul(x, w, p) == (uint32_t)((uint64_t)x * b % p), "u32 mul %u * %u mod %u", x, b, p);
+            }
+        }
+    }
+    // 2^39 is a primitive 64th root of unity (omega_64 for generator 7)
+    CHECK(ref_pow2(1, 39 * 32) == GL_P - 1, "omega_64^32 != -1");
+    printf("field test: %d failures\n", fails);
+    return fails ? 1 : 0;
+}

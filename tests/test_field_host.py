@@ -1,0 +1,17 @@
+"""The device field arithmetic (stntt_amd/csrc/field_*.hpp) compiled for the
+host and checked against 128-bit integer arithmetic on edge and random
+values (tests/cpp/field_test.cpp).  CPU only."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "field_test")
+
+
+def test_field_arithmetic_exact():
+    if not os.path.exists(BIN):
+        subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++20", "-o", BIN,
+                        os.path.join(ROOT, "tests", "cpp", "field_test.cpp")], check=True, capture_output=True)
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
