@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of the same buffer (counter calibration)")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--batch-count", type=int, default=4096)
     return ap.parse_args()
 
 
@@ -71,6 +73,90 @@ def cpu_baseline(log2n: int, budget_s: float) -> dict:
     return {"value": n / med / 1e9, "unit": "GElem/s", "cores": 1, "kind": "port",
             "sample": f"{len(times)} whole Goldilocks 2^{log2n} forward transforms, median "
                       f"{med * 1e3:.0f} ms each, single thread (oracle/ntt_oracle.c)"}
+
+
+def bench_n20(S, ctx, steps: int) -> dict:
+    """2^20, batch 1, forward + inverse, both fields (device-resident)."""
+    out = {}
+    n = 1 << 20
+    rng = np.random.default_rng(3)
+    root, mod = S.find_params(n, 11)
+    for field in ("u32", "goldilocks"):
+        x = rng.integers(0, mod, size=n, dtype=np.uint32) if field == "u32" else \
+            rng.integers(0, P, size=n, dtype=np.uint64)
+        buf = ctx.upload(x)
+
+        def step():
+            if field == "u32":
+                ctx.ntt_u32(buf, 20, 1, root, mod)
+                ctx.ntt_u32(buf, 20, 1, root, mod, inverse=True)
+            else:
+                ctx.ntt_goldilocks(buf, 20, 1)
+                ctx.ntt_goldilocks(buf, 20, 1, inverse=True)
+        for _ in range(5):
+            step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / steps
+        ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
+        out[field] = {"us_per_fwd_inv": round(dt * 1e6, 2), "roundtrip_exact": bool(ok),
+                      "gelem_s": round(2 * n / dt / 1e9, 3)}
+        buf.free()
+    return out
+
+
+def bench_batched(S, ctx, count: int, reps: int) -> dict:
+    """4096 transforms, log2 n ~ U{10..18}, field and direction 50/50: one
+    persistent launch (graph replay) versus per-transform plain launches."""
+    from stntt_amd.workloads import mixed_batch
+    specs = mixed_batch(seed=42, count=count, lo=10, hi=18)
+    rng = np.random.default_rng(4)
+    bufs, descs, elems, nbytes = [], [], 0, 0
+    params = {}
+    for sp in specs:
+        n = 1 << sp.log2n
+        if sp.field == "goldilocks":
+            x, r, m = rng.integers(0, P, size=n, dtype=np.uint64), 0, P
+        else:
+            if sp.log2n not in params:
+                params[sp.log2n] = S.find_params(n, 11)
+            r, m = params[sp.log2n]
+            x = rng.integers(0, m, size=n, dtype=np.uint32)
+        b = ctx.upload(x)
+        bufs.append(b)
+        descs.append((sp.field, sp.inverse, sp.log2n, b, r, m))
+        elems += n
+        nbytes += 2 * x.nbytes
+    batch = S.Batch(ctx, descs)
+    batch.run()
+    ctx.sync()
+    batch.capture()
+    batch.run_graph()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run_graph()
+    ctx.sync()
+    t_batch = (time.perf_counter() - t0) / reps
+    batch.run_plain()
+    ctx.sync()
+    t0 = time.perf_counter()
+    plain_reps = max(1, reps // 4)
+    for _ in range(plain_reps):
+        batch.run_plain()
+    ctx.sync()
+    t_plain = (time.perf_counter() - t0) / plain_reps
+    batch.close()
+    for b in bufs:
+        b.free()
+    return {"transforms": count, "elements": elems, "algorithmic_bytes": nbytes,
+            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+            "batched_transforms_s": round(count / t_batch, 1), "plain_transforms_s": round(count / t_plain, 1),
+            "speedup": round(t_plain / t_batch, 2),
+            "batched_hbm_frac": round(nbytes / t_batch / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def main() -> int:
@@ -153,6 +239,11 @@ def main() -> int:
         "hbm_frac_transform": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "pass_ms": [round(v, 5) for v in mean_pass],
     }
+    if not args.no_extra:
+        log("[bench] n=2^20 forward+inverse, both fields")
+        out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
+        log(f"[bench] batched mixed workload ({args.batch_count} transforms)")
+        out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10)
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)

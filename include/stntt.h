@@ -97,6 +97,40 @@ int stntt_auto_ntt_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint3
 int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_len, uint32_t n,
                               uint32_t batches, int direction, float *kernel_ms);
 
+/* ---- batched mixed transforms -------------------------------------------
+   Thousands of independent transforms of mixed size, field and direction in
+   one persistent launch: resident workgroups pull descriptors from a device
+   work queue (largest first) and each runs its transform to completion.  No
+   counterpart in the reference, whose autoNtt<n> runs one transform (or one
+   same-size batch) per call (src/cuda/ntt/ntt_kernel.cuh:152-201). */
+typedef struct stntt_desc {
+    uint32_t field;     /* stntt_field */
+    uint32_t direction; /* stntt_direction */
+    uint32_t log2n;
+    uint32_t flags;     /* STNTT_SIGNED_INPUT for int32 data */
+    uint32_t root, mod; /* 32-bit field only */
+    const void *src;    /* device buffer of 2^log2n elements */
+    void *dst;          /* may equal src (in place) */
+} stntt_desc;
+
+typedef struct stntt_batch stntt_batch;
+/* Validates every descriptor, makes the twiddle plans resident, orders the
+   queue and uploads it.  The descriptors' buffers must stay valid. */
+int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out);
+/* one persistent launch (preceded by a one-thread queue reset kernel) */
+int stntt_batch_run(stntt_batch *b, stntt_stream stream);
+/* capture the launch into a HIP graph once; later runs replay it */
+int stntt_batch_capture(stntt_batch *b, stntt_stream stream);
+int stntt_batch_run_graph(stntt_batch *b, stntt_stream stream);
+/* the same transforms as one plain per-transform launch sequence each (the
+   reference's calling pattern), for comparison */
+int stntt_batch_run_plain(stntt_batch *b, stntt_stream stream);
+int stntt_batch_destroy(stntt_batch *b);
+/* Tuning: radix split of the batched kernel's sub-steps for (field, log2n),
+   4 bits per sub-step (first in the low bits), as chosen by the autotuner.
+   nsub = 0 restores the built-in split. */
+int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs);
+
 /* ---- device memory and stream plumbing --------------------------------- */
 int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr);
 int stntt_dev_free(stntt_ctx *ctx, void *d_ptr);

@@ -47,6 +47,13 @@ SIGNATURES = [
     ("stntt_ntt_goldilocks", _int, [_vp, _vp, _u32, _u64, _int, _vp]),
     ("stntt_auto_ntt_i32", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
     ("stntt_auto_ntt_goldilocks", _int, [_vp, _vp, _sz, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
+    ("stntt_batch_create", _int, [_vp, _vp, _sz, _c.POINTER(_vp)]),
+    ("stntt_batch_run", _int, [_vp, _vp]),
+    ("stntt_batch_capture", _int, [_vp, _vp]),
+    ("stntt_batch_run_graph", _int, [_vp, _vp]),
+    ("stntt_batch_run_plain", _int, [_vp, _vp]),
+    ("stntt_batch_destroy", _int, [_vp]),
+    ("stntt_set_batch_split", _int, [_vp, _int, _u32, _u32, _u32]),
     ("stntt_dev_alloc", _int, [_vp, _sz, _c.POINTER(_vp)]),
     ("stntt_dev_free", _int, [_vp, _vp]),
     ("stntt_memcpy_htod", _int, [_vp, _vp, _vp, _sz, _vp]),
@@ -60,6 +67,12 @@ SIGNATURES = [
 ]
 
 _lib = None
+
+
+class Desc(_c.Structure):
+    """stntt_desc (include/stntt.h)"""
+    _fields_ = [("field", _u32), ("direction", _u32), ("log2n", _u32), ("flags", _u32),
+                ("root", _u32), ("mod", _u32), ("src", _vp), ("dst", _vp)]
 
 
 class NativeLibraryMissing(RuntimeError):

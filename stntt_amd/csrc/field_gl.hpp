@@ -44,10 +44,12 @@ GL_HD uint64_t gl_add(uint64_t a, uint64_t b) {
 }
 
 // a - b, a, b < P.  A borrow means a - b + 2^64; adding P is subtracting
-// EPS, which cannot borrow again (a - b + 2^64 > 2^32).
+// EPS, which cannot borrow again (a - b + 2^64 > 2^32).  The borrow flag of
+// the subtraction itself selects the correction.
 GL_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
-    uint64_t d = a - b;
-    return a < b ? d - GL_EPS : d;
+    uint64_t d;
+    bool borrow = __builtin_sub_overflow(a, b, &d);
+    return borrow ? d + GL_P : d;
 }
 
 // lo + hi * 2^64 (mod P) for any lo, hi; result is any 64-bit word
@@ -95,6 +97,17 @@ GL_HD uint64_t gl_shl_small_w(uint64_t x, int s) {
     return r;
 }
 
+// The same with a canonical result in one correction: after a carry
+// r + EPS < 2^32 EPS < P is already canonical, and without one r - P =
+// r + EPS (mod 2^64) is due exactly when r >= P, i.e. when r + EPS wraps.
+GL_HD uint64_t gl_shl_small(uint64_t x, int s) {
+    uint64_t lo = x << s;
+    uint64_t h = x >> (64 - s);
+    uint64_t r = h * GL_EPS + lo;
+    uint64_t t = r + GL_EPS;
+    return (r < lo || t < r) ? t : r;
+}
+
 // x * 2^s (mod P) for 0 < s < 64, any x; result is any 64-bit word
 GL_HD uint64_t gl_shl_w(uint64_t x, int s) {
     if (s <= 32) return gl_shl_small_w(x, s);
@@ -110,8 +123,9 @@ GL_HD uint64_t gl_mul_pow2(uint64_t x) {
     constexpr int s = s192 % 96;
     uint64_t r;
     if constexpr (s == 0) r = gl_canon(x);
+    else if constexpr (s <= 32) r = gl_shl_small(x, s);
     else if constexpr (s < 64) r = gl_canon(gl_shl_w(x, s));
-    else r = gl_canon(gl_shl_small_w(gl_shl_w(x, s - 32), 32));
+    else r = gl_shl_small(gl_shl_w(x, s - 32), 32);
     if constexpr (neg) return gl_sub(0, r);
     return r;
 }

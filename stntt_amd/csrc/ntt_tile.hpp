@@ -92,7 +92,8 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(Pa
 #pragma unroll
         for (int k = 0; k < r1; ++k) {
             T x = v[bitrev_c(k, LR1)];
-            if (k != 0 && g != 0) x = FP::mul(x, a.hi[(g * k) << (a.log_rmax - LOGR)], a.rt);
+            // hi[0] = 1: lanes with g = 0 multiply by one rather than diverge
+            if (k != 0) x = FP::mul(x, a.hi[(g * k) << (a.log_rmax - LOGR)], a.rt);
             const uint32_t row = r1 * g + k;
             lds[row * C + (c ^ (row & (C - 1)))] = x;
         }
@@ -123,16 +124,14 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(Pa
         for (int k2 = 0; k2 < r2; ++k2) {
             const uint32_t k = k2 * r1 + g2;
             T x = v[bitrev_c(k2, LR2)];
+            // uniform code: hi[0] = lo[0] = twl[0] = 1
             if constexpr (J1) {
                 const uint32_t e = (t0 + c) * k;  // s = t0 + c, J = 1
-                if (e) {
-                    uint32_t log_b = logn - a.log_rmax;
-                    uint32_t eh = e >> log_b, el = e & ((1u << log_b) - 1);
-                    if (eh) x = FP::mul(x, a.hi[eh], a.rt);
-                    if (el) x = FP::mul(x, a.lo[el], a.rt);
-                }
+                const uint32_t log_b = logn - a.log_rmax;
+                x = FP::mul(x, a.hi[e >> log_b], a.rt);
+                if (log_b) x = FP::mul(x, a.lo[e & ((1u << log_b) - 1)], a.rt);
             } else {
-                if (sJ != 0 && k != 0) x = FP::mul(x, twl[k], a.rt);
+                if (sJ != 0) x = FP::mul(x, twl[k], a.rt);
             }
             if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
             if constexpr (J1) out[(uint64_t)(t0 + c) * R + k] = x;

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "../../include/stntt.h"
+#include "ntt_batch.hpp"
 #include "ntt_pass.hpp"
 #include "plan.hpp"
 
@@ -119,6 +121,22 @@ struct stntt_ctx {
     static constexpr int MAX_PROFILE_EVENTS = 9;
     hipEvent_t prof_ev[MAX_PROFILE_EVENTS] = {};
     int prof_count = 0;  // events recorded by the last profiled transform
+    // autotuned sub-step splits of the batched kernel: (field, log2n) -> (nsub, radix_logs)
+    std::map<std::pair<int, uint32_t>, std::pair<uint32_t, uint32_t>> batch_split;
+    int num_cus = 0;
+};
+
+struct stntt_batch {
+    stntt_ctx *ctx = nullptr;
+    std::vector<stntt_desc> descs;        // as given (for the plain path)
+    stntt::BatchTask *d_tasks = nullptr;  // queue, largest first
+    uint32_t count = 0;
+    uint32_t *d_counter = nullptr;
+    uint64_t *d_scratch = nullptr;
+    uint64_t scratch_stride = 0;  // 64-bit words per workgroup
+    uint32_t grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace stntt {
@@ -369,6 +387,119 @@ static int auto_ntt_common(stntt_ctx *ctx, T *host_vec, size_t host_len, uint32_
     return STNTT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batched mixed transforms
+// ---------------------------------------------------------------------------
+// Default sub-step split for the batched kernel: radix 16 where possible,
+// the remainder spread evenly; the count is made even for transforms that
+// stream through global memory so the ping-pong ends in dst.
+static void default_batch_split(uint32_t L, bool global_path, uint32_t *nsub, uint32_t *radix_logs) {
+    uint32_t ns = L == 0 ? 0 : (L + 3) / 4;
+    if (global_path && (ns & 1)) ns += 1;
+    uint32_t logs = 0;
+    for (uint32_t i = 0; i < ns; ++i) logs |= (L / ns + (i < L % ns ? 1u : 0u)) << (4 * i);
+    *nsub = ns;
+    *radix_logs = logs;
+}
+
+static bool batch_global_path(int field, uint32_t L) {
+    uint64_t w = field == STNTT_FIELD_GOLDILOCKS ? 8 : 4;
+    return (w << L) > (uint64_t)BATCH_LDS_BYTES;
+}
+
+static int batch_validate_split(uint32_t L, bool global_path, uint32_t nsub, uint32_t logs) {
+    if (nsub == 0 || nsub > 8) return fail(STNTT_EINVAL, "bad sub-step count %u", nsub);
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < nsub; ++i) {
+        uint32_t lr = (logs >> (4 * i)) & 15;
+        if (lr < 1 || lr > 4) return fail(STNTT_EINVAL, "sub-step radix 2^%u outside [2, 16]", lr);
+        sum += lr;
+    }
+    if (sum != L) return fail(STNTT_EINVAL, "sub-step radices cover 2^%u, not 2^%u", sum, L);
+    if (global_path && (nsub & 1)) return fail(STNTT_EINVAL, "streamed transforms need an even sub-step count");
+    return STNTT_OK;
+}
+
+static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch *b) {
+    std::vector<BatchTask> tasks(count);
+    uint64_t max_stream = 0;  // largest streamed transform, in 64-bit words
+    for (size_t i = 0; i < count; ++i) {
+        const stntt_desc &d = descs[i];
+        BatchTask &t = tasks[i];
+        if (d.field != STNTT_FIELD_U32 && d.field != STNTT_FIELD_GOLDILOCKS)
+            return fail(STNTT_EINVAL, "descriptor %zu: bad field %u", i, d.field);
+        if (d.direction > 1) return fail(STNTT_EINVAL, "descriptor %zu: bad direction %u", i, d.direction);
+        if (d.log2n > 18) return fail(STNTT_EINVAL, "descriptor %zu: log2n %u above the batched limit 18", i, d.log2n);
+        if (!d.src || !d.dst) return fail(STNTT_EINVAL, "descriptor %zu: null buffer", i);
+        const Plan *plan;
+        int rc;
+        if (d.field == STNTT_FIELD_U32) {
+            if ((rc = check_u32_params(d.log2n, d.root, d.mod, d.direction))) return rc;
+            if ((rc = get_plan(ctx, STNTT_FIELD_U32, d.log2n, d.root % d.mod, d.mod, d.direction, &plan))) return rc;
+            t.rt32 = plan->rt_u32;
+        } else {
+            if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, d.log2n, 0, GL_P, d.direction, &plan))) return rc;
+            t.rtgl = plan->rt_gl;
+        }
+        t.src = d.src;
+        t.dst = d.dst;
+        t.hi = plan->hi();
+        t.lo = plan->lo();
+        t.field = d.field;
+        t.dir = d.direction;
+        t.log_n = d.log2n;
+        t.log_rmax = plan->log_rmax;
+        t.flags = PASS_FIRST | PASS_LAST | (d.direction == STNTT_INVERSE ? PASS_SCALE : 0) |
+                  ((d.field == STNTT_FIELD_U32 && (d.flags & STNTT_SIGNED_INPUT)) ? PASS_SIGNED : 0);
+        const bool gp = batch_global_path(d.field, d.log2n);
+        auto it = ctx->batch_split.find({(int)d.field, d.log2n});
+        if (it != ctx->batch_split.end()) {
+            t.nsub = it->second.first;
+            t.radix_logs = it->second.second;
+        } else {
+            default_batch_split(d.log2n, gp, &t.nsub, &t.radix_logs);
+        }
+        if (d.log2n > 0 && (rc = batch_validate_split(d.log2n, gp, t.nsub, t.radix_logs))) return rc;
+        if (gp) max_stream = std::max<uint64_t>(max_stream, 1ull << d.log2n);
+    }
+    // largest first (longest-processing-time order) so the tail is short
+    std::stable_sort(tasks.begin(), tasks.end(), [](const BatchTask &x, const BatchTask &y) {
+        uint64_t cx = ((uint64_t)(x.field ? 2 : 1) << x.log_n) * (x.log_n + 1);
+        uint64_t cy = ((uint64_t)(y.field ? 2 : 1) << y.log_n) * (y.log_n + 1);
+        return cx > cy;
+    });
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_batch_kernel, BATCH_THREADS, 0));
+    if (per_cu < 1) per_cu = 1;
+    b->grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, std::max<size_t>(count, 1));
+    b->count = (uint32_t)count;
+    b->scratch_stride = max_stream;
+    HIP_TRY(hipMalloc(&b->d_tasks, std::max<size_t>(count, 1) * sizeof(BatchTask)));
+    if (count) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), count * sizeof(BatchTask), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&b->d_counter, sizeof(uint32_t)));
+    if (max_stream) HIP_TRY(hipMalloc(&b->d_scratch, (size_t)b->grid * max_stream * sizeof(uint64_t)));
+    b->descs.assign(descs, descs + count);
+    return STNTT_OK;
+}
+
+static int batch_launch(stntt_batch *b, hipStream_t s) {
+    hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, b->d_counter);
+    HIP_TRY(hipGetLastError());
+    if (b->count == 0) return STNTT_OK;
+    hipLaunchKernelGGL(ntt_batch_kernel, dim3(b->grid), dim3(BATCH_THREADS), 0, s, b->d_tasks, b->count,
+                       b->d_counter, b->d_scratch, b->scratch_stride);
+    HIP_TRY(hipGetLastError());
+    return STNTT_OK;
+}
+
+static void batch_free(stntt_batch *b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_tasks) (void)hipFree(b->d_tasks);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->d_scratch) (void)hipFree(b->d_scratch);
+}
+
 }  // namespace stntt
 
 using namespace stntt;
@@ -390,6 +521,7 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
     auto ctx = std::make_unique<stntt_ctx>();
     ctx->device = device;
     HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&ctx->ev_start));
     HIP_TRY(hipEventCreate(&ctx->ev_stop));
@@ -539,6 +671,96 @@ int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_le
     return auto_ntt_common<uint64_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, uint64_t *d) {
         return stntt_ntt_goldilocks(ctx, d, L, batches, direction, nullptr);
     });
+}
+
+int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out) {
+    if (!ctx || !out || (count && !descs)) return fail(STNTT_EINVAL, "null argument");
+    if (count > 0xFFFFFFFFull) return fail(STNTT_EINVAL, "too many descriptors");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    auto b = std::make_unique<stntt_batch>();
+    b->ctx = ctx;
+    if ((rc = batch_build(ctx, descs, count, b.get()))) {
+        batch_free(b.get());
+        return rc;
+    }
+    *out = b.release();
+    return STNTT_OK;
+}
+
+int stntt_batch_run(stntt_batch *b, stntt_stream stream) {
+    if (!b) return fail(STNTT_EINVAL, "null batch");
+    int rc;
+    if ((rc = set_device(b->ctx))) return rc;
+    return batch_launch(b, stream ? (hipStream_t)stream : b->ctx->stream);
+}
+
+int stntt_batch_capture(stntt_batch *b, stntt_stream stream) {
+    if (!b) return fail(STNTT_EINVAL, "null batch");
+    int rc;
+    if ((rc = set_device(b->ctx))) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+    if (b->graph_exec) return STNTT_OK;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch(b, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(STNTT_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+    return STNTT_OK;
+}
+
+int stntt_batch_run_graph(stntt_batch *b, stntt_stream stream) {
+    if (!b) return fail(STNTT_EINVAL, "null batch");
+    if (!b->graph_exec) return fail(STNTT_EINVAL, "batch not captured");
+    int rc;
+    if ((rc = set_device(b->ctx))) return rc;
+    HIP_TRY(hipGraphLaunch(b->graph_exec, stream ? (hipStream_t)stream : b->ctx->stream));
+    return STNTT_OK;
+}
+
+int stntt_batch_run_plain(stntt_batch *b, stntt_stream stream) {
+    if (!b) return fail(STNTT_EINVAL, "null batch");
+    int rc;
+    if ((rc = set_device(b->ctx))) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+    for (const stntt_desc &d : b->descs) {
+        size_t bytes = (size_t)(d.field == STNTT_FIELD_GOLDILOCKS ? 8 : 4) << d.log2n;
+        if (d.src != d.dst) HIP_TRY(hipMemcpyAsync(d.dst, d.src, bytes, hipMemcpyDeviceToDevice, s));
+        if (d.field == STNTT_FIELD_GOLDILOCKS) rc = stntt_ntt_goldilocks(b->ctx, (uint64_t *)d.dst, d.log2n, 1,
+                                                                         d.direction, s);
+        else rc = stntt_ntt_u32(b->ctx, (uint32_t *)d.dst, d.log2n, 1, d.root, d.mod, d.direction, d.flags, s);
+        if (rc) return rc;
+    }
+    return STNTT_OK;
+}
+
+int stntt_batch_destroy(stntt_batch *b) {
+    if (!b) return STNTT_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_free(b);
+    delete b;
+    return STNTT_OK;
+}
+
+int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (field != STNTT_FIELD_U32 && field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");
+    if (log2n > 18) return fail(STNTT_EINVAL, "log2n %u above the batched limit 18", log2n);
+    if (nsub == 0) {
+        ctx->batch_split.erase({field, log2n});
+        return STNTT_OK;
+    }
+    int rc = batch_validate_split(log2n, batch_global_path(field, log2n), nsub, radix_logs);
+    if (rc) return rc;
+    ctx->batch_split[{field, log2n}] = {nsub, radix_logs};
+    return STNTT_OK;
 }
 
 int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr) {

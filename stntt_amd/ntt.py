@@ -243,3 +243,63 @@ def ntt_goldilocks(vec: np.ndarray, size: int, batches: int = 1, inverse: bool =
     """Goldilocks transform of vec[batches][size] (uint64) on the GPU, in place."""
     auto_ntt(NttArgs(vec, size, batches, 0, GL_P, field="goldilocks", inverse=inverse), ctx)
     return vec
+
+
+# --------------------------------------------------------------------------
+# batched mixed transforms
+# --------------------------------------------------------------------------
+class Batch:
+    """Thousands of independent transforms in one persistent launch.
+
+    descs: iterable of (field, inverse, log2n, buf, root, mod) where field is
+    "u32" or "goldilocks", buf a DeviceBuffer (transformed in place) and
+    root/mod the 32-bit field parameters (ignored for Goldilocks).
+    """
+
+    def __init__(self, ctx: Context, descs):
+        self.ctx = ctx
+        arr = (_lib.Desc * max(len(descs), 1))()
+        for i, (field, inverse, log2n, buf, root, mod) in enumerate(descs):
+            ptr = buf.ptr if isinstance(buf, DeviceBuffer) else buf
+            arr[i] = _lib.Desc(_lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32,
+                               _lib.INVERSE if inverse else _lib.FORWARD, log2n, 0,
+                               root or 0, mod or 0, ptr, ptr)
+        h = ctypes.c_void_p()
+        check(lib().stntt_batch_create(ctx.handle, arr, len(descs), ctypes.byref(h)))
+        self.handle = h
+        self.count = len(descs)
+
+    def run(self, stream: int | None = None) -> None:
+        check(lib().stntt_batch_run(self.handle, stream))
+
+    def capture(self, stream: int | None = None) -> None:
+        check(lib().stntt_batch_capture(self.handle, stream))
+
+    def run_graph(self, stream: int | None = None) -> None:
+        check(lib().stntt_batch_run_graph(self.handle, stream))
+
+    def run_plain(self, stream: int | None = None) -> None:
+        check(lib().stntt_batch_run_plain(self.handle, stream))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().stntt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_batch_split(ctx: Context, field: str, log2n: int, radix_logs: list[int] | None) -> None:
+    """Install an autotuned sub-step split for the batched kernel."""
+    f = _lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32
+    if not radix_logs:
+        check(lib().stntt_set_batch_split(ctx.handle, f, log2n, 0, 0))
+        return
+    packed = 0
+    for i, lr in enumerate(radix_logs):
+        packed |= lr << (4 * i)
+    check(lib().stntt_set_batch_split(ctx.handle, f, log2n, len(radix_logs), packed))

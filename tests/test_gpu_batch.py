@@ -1,0 +1,159 @@
+"""Batched mixed transforms: one persistent launch over a descriptor queue.
+
+Every transform in a mixed batch must equal the oracle bit for bit, equal
+the same transform run alone (plain launches), and stay exact across
+repeated hipGraph replays with other GPU work in between.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import stntt_amd as S
+from oracle import oracle as O
+from stntt_amd import _lib
+from stntt_amd.workloads import mixed_batch
+
+pytestmark = pytest.mark.gpu
+P = S.GL_P
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = S.Context(0)
+    yield c
+    c.close()
+
+
+def make_inputs(specs, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for sp in specs:
+        n = 1 << sp.log2n
+        if sp.field == "goldilocks":
+            out.append((rng.integers(0, 2**64, size=n, dtype=np.uint64), 0, P))
+        else:
+            root, mod = S.find_params(n, 11)
+            out.append((rng.integers(0, mod, size=n, dtype=np.uint32), root, mod))
+    return out
+
+
+def oracle_apply(sp, x, root, mod, times=1):
+    y = x
+    for _ in range(times):
+        if sp.field == "goldilocks":
+            y = O.inverse_gl(y, sp.log2n) if sp.inverse else O.stockham_gl(y, sp.log2n)
+        else:
+            n = 1 << sp.log2n
+            y = O.inverse_u32(y, n, root, mod) if sp.inverse else O.stockham_u32(y, n, root, mod)
+    return y
+
+
+def upload_all(ctx, inputs):
+    return [ctx.upload(x) for x, _, _ in inputs]
+
+
+def descs_of(specs, inputs, bufs):
+    return [(sp.field, sp.inverse, sp.log2n, b, r, m) for sp, (_, r, m), b in zip(specs, inputs, bufs)]
+
+
+def check_all(specs, inputs, bufs, times=1, tag=""):
+    for i, (sp, (x, r, m), b) in enumerate(zip(specs, inputs, bufs)):
+        got = b.copy_to(np.empty_like(x))
+        want = oracle_apply(sp, x, r, m, times)
+        assert np.array_equal(got, want), f"{tag} transform {i}: {sp}"
+
+
+def test_mixed_batch_matches_oracle(ctx):
+    specs = mixed_batch(seed=7, count=96, lo=1, hi=18)
+    inputs = make_inputs(specs, 1)
+    bufs = upload_all(ctx, inputs)
+    b = S.Batch(ctx, descs_of(specs, inputs, bufs))
+    print("batch of", len(specs), flush=True)
+    b.run()
+    ctx.sync()
+    check_all(specs, inputs, bufs)
+    b.close()
+
+
+def test_batch_equals_plain_launches(ctx):
+    """Each result is the same alone (per-transform launches) and in the mix."""
+    specs = mixed_batch(seed=9, count=48, lo=10, hi=18)
+    inputs = make_inputs(specs, 2)
+    bufs_mix = upload_all(ctx, inputs)
+    bufs_alone = upload_all(ctx, inputs)
+    S.Batch(ctx, descs_of(specs, inputs, bufs_mix)).run()
+    plain = S.Batch(ctx, descs_of(specs, inputs, bufs_alone))
+    plain.run_plain()
+    ctx.sync()
+    for i, (bm, ba, (x, _, _)) in enumerate(zip(bufs_mix, bufs_alone, inputs)):
+        assert np.array_equal(bm.copy_to(np.empty_like(x)), ba.copy_to(np.empty_like(x))), i
+
+
+def test_graph_replays_stay_exact(ctx):
+    """Replays of the captured launch, with eager work and syncs between
+    them, each advance every transform exactly once more."""
+    specs = mixed_batch(seed=11, count=40, lo=10, hi=16)
+    inputs = make_inputs(specs, 3)
+    bufs = upload_all(ctx, inputs)
+    b = S.Batch(ctx, descs_of(specs, inputs, bufs))
+    b.capture()
+    side = ctx.upload(np.arange(1 << 12, dtype=np.uint64))
+    for rep in range(1, 4):
+        b.run_graph()
+        ctx.ntt_goldilocks(side, 12, 1)  # unrelated eager work between replays
+        ctx.sync()
+        check_all(specs, inputs, bufs, times=rep, tag=f"replay {rep}")
+    b.close()
+
+
+@pytest.mark.parametrize("split", [[4, 4, 4, 4], [3, 3, 3, 3, 2, 2], [4, 4, 2, 2, 2, 2]])
+def test_tuned_splits_same_result(ctx, split):
+    L = 16
+    S.set_batch_split(ctx, "goldilocks", L, split)
+    try:
+        x = np.random.default_rng(5).integers(0, 2**64, size=1 << L, dtype=np.uint64)
+        buf = ctx.upload(x)
+        S.Batch(ctx, [("goldilocks", False, L, buf, 0, P)]).run()
+        assert np.array_equal(buf.copy_to(np.empty_like(x)), O.stockham_gl(x, L))
+    finally:
+        S.set_batch_split(ctx, "goldilocks", L, None)
+
+
+def test_out_of_place_descriptors(ctx):
+    rng = np.random.default_rng(13)
+    xs, srcs, dsts, arr = [], [], [], (_lib.Desc * 3)()
+    for i, (field, L) in enumerate([(1, 9), (0, 14), (1, 17)]):
+        n = 1 << L
+        if field:
+            x, root, mod = rng.integers(0, 2**64, size=n, dtype=np.uint64), 0, 0
+        else:
+            root, mod = S.find_params(n, 11)
+            x = rng.integers(0, mod, size=n, dtype=np.uint32)
+        src, dst = ctx.upload(x), ctx.alloc(x.nbytes)
+        xs.append((field, L, x, root, mod))
+        srcs.append(src)
+        dsts.append(dst)
+        arr[i] = _lib.Desc(field, 0, L, 0, root, mod, src.ptr, dst.ptr)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().stntt_batch_create(ctx.handle, arr, 3, ctypes.byref(h)))
+    _lib.check(_lib.lib().stntt_batch_run(h, None))
+    ctx.sync()
+    for (field, L, x, root, mod), src, dst in zip(xs, srcs, dsts):
+        got = dst.copy_to(np.empty_like(x))
+        want = O.stockham_gl(x, L) if field else O.stockham_u32(x, 1 << L, root, mod)
+        assert np.array_equal(got, want)
+        assert np.array_equal(src.copy_to(np.empty_like(x)), x)  # source untouched
+    _lib.lib().stntt_batch_destroy(h)
+
+
+def test_batch_rejects_bad_descriptors(ctx):
+    buf = ctx.alloc(8 << 19)
+    with pytest.raises(ValueError):
+        S.Batch(ctx, [("goldilocks", False, 19, buf, 0, P)])   # above the batched limit
+    with pytest.raises(ValueError):
+        S.Batch(ctx, [("u32", False, 4, buf, 2, 17)])          # 2 is not of order 16 mod 17
+    with pytest.raises(ValueError):
+        S.set_batch_split(ctx, "goldilocks", 16, [4, 4, 4, 3, 1])  # odd count on a streamed size
