@@ -29,6 +29,8 @@ import time
 
 import numpy as np
 
+from stntt_amd import _lib
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -51,6 +53,7 @@ def parse():
                     help="also run a plain device copy of the same buffer (counter calibration)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-count", type=int, default=400)
     return ap.parse_args()
 
 
@@ -159,6 +162,79 @@ def bench_batched(S, ctx, count: int, reps: int) -> dict:
             "batched_hbm_frac": round(nbytes / t_batch / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def bench_serving(S, count: int, workers: int) -> dict:
+    """Poisson arrivals, Goldilocks, log2 n ~ U{12..22}, direction 50/50,
+    64 GB admission budget: the continuous-batching server versus plain
+    sequential launches (one request at a time, FIFO) on the same trace."""
+    from stntt_amd.workloads import poisson_arrivals
+    srv = S.Server(0, hbm_budget_bytes=64 << 30, workers=workers)
+    rng = np.random.default_rng(5)
+    trace0 = poisson_arrivals(seed=17, count=count, rate_per_s=1.0)
+    bufs = []
+    for _, sp in trace0:
+        a = srv.pinned(1 << sp.log2n, np.uint64)
+        a.array[:] = rng.integers(0, P, size=1 << sp.log2n, dtype=np.uint64)
+        bufs.append(a)
+    # plain sequential service time (H2D, transform, D2H, synchronous)
+    ctx = S.Context(0)
+    dev = ctx.alloc(8 << 22)
+
+    def serve_plain(sp, a):
+        nbytes = 8 << sp.log2n
+        _lib.check(_lib.lib().stntt_memcpy_htod(ctx.handle, dev.ptr, a.ptr, nbytes, None))
+        ctx.ntt_goldilocks(dev.ptr, sp.log2n, 1, inverse=sp.inverse)
+        _lib.check(_lib.lib().stntt_memcpy_dtoh(ctx.handle, a.ptr, dev.ptr, nbytes, None))
+    for (_, sp), a in zip(trace0[:20], bufs[:20]):
+        serve_plain(sp, a)
+    t0 = time.perf_counter()
+    for (_, sp), a in zip(trace0, bufs):
+        serve_plain(sp, a)
+    seq_capacity = count / (time.perf_counter() - t0)
+
+    def replay(rate, use_server):
+        scale = 1.0 / rate  # trace0 has rate 1/s
+        lat, ids = [], []
+        start = time.perf_counter()
+        for (t, sp), a in zip(trace0, bufs):
+            due = start + t * scale
+            while time.perf_counter() < due:
+                pass
+            if use_server:
+                ids.append((srv.submit("goldilocks", sp.log2n, a, inverse=sp.inverse), due))
+            else:
+                serve_plain(sp, a)
+                lat.append((time.perf_counter() - due) * 1e3)
+        if use_server:
+            for rid, _ in ids:
+                srv.wait(rid)
+        span = time.perf_counter() - start
+        return lat, span
+
+    out = {"requests": count, "workers": workers, "budget_bytes": 64 << 30,
+           "plain_sequential_capacity_tps": round(seq_capacity, 1), "rates": []}
+    for frac in (0.8, 2.0):
+        rate = frac * seq_capacity
+        lat_plain, span_plain = replay(rate, False)
+        srv.reset_stats()
+        _, span_srv = replay(rate, True)
+        st = srv.stats()
+        lp = np.array(lat_plain)
+        out["rates"].append({
+            "offered_tps": round(rate, 1),
+            "server": {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                       "transforms_s": round(count / span_srv, 1), "rounds": st["rounds"],
+                       "batched_requests": st["batched"], "peak_bytes": st["peak_bytes"]},
+            "plain_sequential": {"p50_ms": round(float(np.percentile(lp, 50)), 3),
+                                 "p99_ms": round(float(np.percentile(lp, 99)), 3),
+                                 "transforms_s": round(count / span_plain, 1)}})
+    dev.free()
+    ctx.close()
+    for a in bufs:
+        a.free()
+    srv.close()
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -244,6 +320,8 @@ def main() -> int:
         out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
         log(f"[bench] batched mixed workload ({args.batch_count} transforms)")
         out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10)
+        log("[bench] serving stream")
+        out["serving"] = bench_serving(S, args.serve_count, 4)
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)

@@ -131,6 +131,49 @@ int stntt_batch_destroy(stntt_batch *b);
    nsub = 0 restores the built-in split. */
 int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs);
 
+/* ---- serving: continuous batching under an HBM budget -------------------
+   No counterpart in the reference, whose drivers run one synchronous
+   transform at a time (src/bench.cu:26-55).  Requests are queued from any
+   thread; a server thread admits them in arrival order while the device
+   bytes they hold stay within the budget, launches each admission round on a
+   free worker stream (small transforms of the round in one persistent batch
+   launch, larger ones on the multi-pass path) and retires finished rounds,
+   so requests that arrive while others run are admitted as memory frees up
+   and their work overlaps on the device.  Twiddles stay resident per worker.
+   Host buffers must be pinned (stntt_server_host_alloc). */
+typedef struct stntt_request {
+    uint32_t field, direction, log2n, flags;
+    uint32_t root, mod;    /* 32-bit field only */
+    const void *host_in;   /* pinned, 2^log2n elements */
+    void *host_out;        /* pinned, may equal host_in */
+} stntt_request;
+
+typedef struct stntt_server_stats {
+    uint64_t completed;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result in host_out */
+    double span_s;                          /* first submit -> last completion */
+    uint64_t peak_bytes;                    /* peak device bytes admitted */
+    uint64_t rounds;                        /* admission rounds launched */
+    uint64_t batched;                       /* requests run through the batch kernel */
+} stntt_server_stats;
+
+typedef struct stntt_server stntt_server;
+/* workers: concurrent streams (1..16).  The budget covers admitted request
+   buffers plus a fixed per-worker reservation for scratch. */
+int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out);
+int stntt_server_destroy(stntt_server *srv);
+int stntt_server_host_alloc(stntt_server *srv, size_t bytes, void **host_ptr);
+int stntt_server_host_free(stntt_server *srv, void *host_ptr);
+/* queue a request; never blocks.  A request that cannot fit the budget even
+   alone is STNTT_EINVAL. */
+int stntt_server_submit(stntt_server *srv, const stntt_request *req, uint64_t *id);
+/* block until the request is done; returns its latency and forgets it.
+   Unknown ids are STNTT_ENOTFOUND. */
+int stntt_server_wait(stntt_server *srv, uint64_t id, double *latency_ms);
+int stntt_server_poll(stntt_server *srv, uint64_t id, int *done);
+int stntt_server_stats_get(stntt_server *srv, stntt_server_stats *out);
+int stntt_server_reset_stats(stntt_server *srv);
+
 /* ---- device memory and stream plumbing --------------------------------- */
 int stntt_dev_alloc(stntt_ctx *ctx, size_t bytes, void **d_ptr);
 int stntt_dev_free(stntt_ctx *ctx, void *d_ptr);

@@ -54,6 +54,15 @@ SIGNATURES = [
     ("stntt_batch_run_plain", _int, [_vp, _vp]),
     ("stntt_batch_destroy", _int, [_vp]),
     ("stntt_set_batch_split", _int, [_vp, _int, _u32, _u32, _u32]),
+    ("stntt_server_create", _int, [_int, _u64, _u32, _c.POINTER(_vp)]),
+    ("stntt_server_destroy", _int, [_vp]),
+    ("stntt_server_host_alloc", _int, [_vp, _sz, _c.POINTER(_vp)]),
+    ("stntt_server_host_free", _int, [_vp, _vp]),
+    ("stntt_server_submit", _int, [_vp, _vp, _c.POINTER(_u64)]),
+    ("stntt_server_wait", _int, [_vp, _u64, _c.POINTER(_c.c_double)]),
+    ("stntt_server_poll", _int, [_vp, _u64, _c.POINTER(_int)]),
+    ("stntt_server_stats_get", _int, [_vp, _vp]),
+    ("stntt_server_reset_stats", _int, [_vp]),
     ("stntt_dev_alloc", _int, [_vp, _sz, _c.POINTER(_vp)]),
     ("stntt_dev_free", _int, [_vp, _vp]),
     ("stntt_memcpy_htod", _int, [_vp, _vp, _vp, _sz, _vp]),
@@ -73,6 +82,19 @@ class Desc(_c.Structure):
     """stntt_desc (include/stntt.h)"""
     _fields_ = [("field", _u32), ("direction", _u32), ("log2n", _u32), ("flags", _u32),
                 ("root", _u32), ("mod", _u32), ("src", _vp), ("dst", _vp)]
+
+
+class Request(_c.Structure):
+    """stntt_request (include/stntt.h)"""
+    _fields_ = [("field", _u32), ("direction", _u32), ("log2n", _u32), ("flags", _u32),
+                ("root", _u32), ("mod", _u32), ("host_in", _vp), ("host_out", _vp)]
+
+
+class ServerStats(_c.Structure):
+    """stntt_server_stats (include/stntt.h)"""
+    _fields_ = [("completed", _u64), ("p50_ms", _c.c_double), ("p99_ms", _c.c_double), ("mean_ms", _c.c_double),
+                ("max_ms", _c.c_double), ("span_s", _c.c_double), ("peak_bytes", _u64), ("rounds", _u64),
+                ("batched", _u64)]
 
 
 class NativeLibraryMissing(RuntimeError):

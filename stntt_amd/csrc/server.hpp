@@ -1,0 +1,96 @@
+// server.hpp -- continuous-batching transform server (host side).
+//
+// No counterpart in the reference (its drivers call autoNtt synchronously,
+// one transform at a time: src/bench.cu:26-55, src/main.cu:37-60).
+//
+// A server owns W worker contexts (one HIP stream, scratch pool and resident
+// twiddle plans each) and a device-buffer cache in power-of-two size classes.
+// Clients submit requests (field, direction, log2 n, pinned host in/out) from
+// any thread; a server thread
+//   1. admits pending requests in arrival order while the device bytes held
+//      by admitted requests stay within the HBM budget;
+//   2. launches each admission round on the least-loaded worker stream: the
+//      round's small transforms (log2 n <= SERVER_BATCH_MAX_LOG) go into one
+//      persistent descriptor-queue launch, larger ones run the multi-pass
+//      path; uploads, kernels and downloads are asynchronous on that stream,
+//      so rounds on different workers overlap on the device;
+//   3. retires rounds whose completion event has fired, returns their bytes
+//      to the budget and their buffers to the cache, and wakes waiters.
+// Requests arriving while others run are admitted as soon as budget frees
+// up: continuous batching, not a queue drained one request at a time.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/stntt.h"
+#include "ntt_batch.hpp"
+
+namespace stntt {
+
+constexpr uint32_t SERVER_BATCH_MAX_LOG = 16;   // largest size sent through the batch kernel
+constexpr uint32_t SERVER_BATCH_GRID = 256;     // workgroups of one round's batch launch
+constexpr uint32_t SERVER_MAX_ROUND = 256;      // requests per admission round
+
+using Clock = std::chrono::steady_clock;
+
+struct ServerRequest {
+    uint64_t id;
+    stntt_request req;
+    Clock::time_point t_submit;
+    double latency_ms = 0;
+    int status = 0;
+    bool done = false;
+    void *d_buf = nullptr;
+    size_t d_bytes = 0;  // size-class bytes of d_buf
+    uint64_t cost = 0;   // bytes charged against the budget
+};
+
+struct ServerWorker {
+    stntt_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    BatchTask *h_tasks = nullptr;  // pinned staging of the round's descriptors
+    BatchTask *d_tasks = nullptr;
+    uint32_t *d_counter = nullptr;
+    uint64_t *d_scratch = nullptr;  // SERVER_BATCH_GRID slices of 2^SERVER_BATCH_MAX_LOG words
+    int inflight = 0;
+};
+
+struct ServerRound {
+    int worker;
+    hipEvent_t done_ev;
+    std::vector<uint64_t> ids;
+};
+
+}  // namespace stntt
+
+struct stntt_server {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    std::vector<stntt::ServerWorker> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> pending;  // arrival order
+    std::unordered_map<uint64_t, stntt::ServerRequest> reqs;
+    std::deque<stntt::ServerRound> rounds;
+    std::map<size_t, std::vector<void *>> cache;  // size class -> free device buffers
+    std::vector<hipEvent_t> free_events;
+    std::vector<double> latencies;
+    uint64_t next_id = 1;
+    uint64_t completed = 0;
+    uint64_t peak_used = 0;
+    uint64_t rounds_launched = 0, batched_requests = 0;
+    bool stop = false;
+    std::thread thread;
+    stntt::Clock::time_point t_first{}, t_last{};
+    bool any = false;
+};

@@ -18,6 +18,7 @@
 #include "ntt_batch.hpp"
 #include "ntt_pass.hpp"
 #include "plan.hpp"
+#include "server.hpp"
 
 namespace stntt {
 
@@ -500,6 +501,189 @@ static void batch_free(stntt_batch *b) {
     if (b->d_scratch) (void)hipFree(b->d_scratch);
 }
 
+// ---------------------------------------------------------------------------
+// serving
+// ---------------------------------------------------------------------------
+static uint64_t elem_bytes(uint32_t field) { return field == STNTT_FIELD_GOLDILOCKS ? 8 : 4; }
+
+static int server_check_request(const stntt_request &r) {
+    if (r.field != STNTT_FIELD_U32 && r.field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");
+    if (r.direction > 1) return fail(STNTT_EINVAL, "bad direction");
+    if (!r.host_in || !r.host_out) return fail(STNTT_EINVAL, "null host buffer");
+    if (r.field == STNTT_FIELD_GOLDILOCKS) {
+        if (r.log2n > 28) return fail(STNTT_EINVAL, "log2n %u too large for serving", r.log2n);
+        return STNTT_OK;
+    }
+    if (r.log2n > 28) return fail(STNTT_EINVAL, "log2n %u too large for serving", r.log2n);
+    return check_u32_params(r.log2n, r.root, r.mod, r.direction);
+}
+
+// per-worker scratch reserved up front: multi-pass ping-pong for the largest
+// served size plus the batch kernel's streaming slices
+static uint64_t server_worker_reserve(uint32_t max_log) {
+    return 2 * (8ull << max_log) + (uint64_t)SERVER_BATCH_GRID * (8ull << SERVER_BATCH_MAX_LOG);
+}
+constexpr uint32_t SERVER_MAX_LOG = 22;
+
+static void *server_get_buffer(stntt_server *S, size_t bytes) {
+    auto &v = S->cache[bytes];
+    if (!v.empty()) {
+        void *p = v.back();
+        v.pop_back();
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    return p;
+}
+
+// Launch one admission round on worker w.  Runs on the server thread with
+// the lock released; the requests it touches are owned by the round.
+static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest *> &round, hipEvent_t ev) {
+    ServerWorker &W = S->workers[w];
+    hipStream_t s = W.stream;
+    uint32_t nb = 0;
+    for (ServerRequest *r : round) {
+        const stntt_request &q = r->req;
+        const size_t bytes = (size_t)elem_bytes(q.field) << q.log2n;
+        HIP_TRY(hipMemcpyAsync(r->d_buf, q.host_in, bytes, hipMemcpyHostToDevice, s));
+        if (q.log2n <= SERVER_BATCH_MAX_LOG) {
+            BatchTask &t = W.h_tasks[nb++];
+            t = BatchTask{};
+            const Plan *plan;
+            int rc;
+            if (q.field == STNTT_FIELD_U32) {
+                if ((rc = get_plan(W.ctx, STNTT_FIELD_U32, q.log2n, q.root % q.mod, q.mod, q.direction, &plan))) return rc;
+                t.rt32 = plan->rt_u32;
+            } else {
+                if ((rc = get_plan(W.ctx, STNTT_FIELD_GOLDILOCKS, q.log2n, 0, GL_P, q.direction, &plan))) return rc;
+                t.rtgl = plan->rt_gl;
+            }
+            t.src = r->d_buf;
+            t.dst = r->d_buf;
+            t.hi = plan->hi();
+            t.lo = plan->lo();
+            t.field = q.field;
+            t.dir = q.direction;
+            t.log_n = q.log2n;
+            t.log_rmax = plan->log_rmax;
+            t.flags = PASS_FIRST | PASS_LAST | (q.direction == STNTT_INVERSE ? PASS_SCALE : 0) |
+                      ((q.field == STNTT_FIELD_U32 && (q.flags & STNTT_SIGNED_INPUT)) ? PASS_SIGNED : 0);
+            default_batch_split(q.log2n, batch_global_path(q.field, q.log2n), &t.nsub, &t.radix_logs);
+        }
+    }
+    if (nb) {
+        HIP_TRY(hipMemcpyAsync(W.d_tasks, W.h_tasks, nb * sizeof(BatchTask), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, W.d_counter);
+        hipLaunchKernelGGL(ntt_batch_kernel, dim3(std::min<uint32_t>(nb, SERVER_BATCH_GRID)), dim3(BATCH_THREADS), 0, s,
+                           W.d_tasks, nb, W.d_counter, W.d_scratch, (uint64_t)1 << SERVER_BATCH_MAX_LOG);
+        HIP_TRY(hipGetLastError());
+    }
+    for (ServerRequest *r : round) {
+        const stntt_request &q = r->req;
+        if (q.log2n <= SERVER_BATCH_MAX_LOG) continue;
+        int rc = q.field == STNTT_FIELD_GOLDILOCKS
+                     ? stntt_ntt_goldilocks(W.ctx, (uint64_t *)r->d_buf, q.log2n, 1, q.direction, s)
+                     : stntt_ntt_u32(W.ctx, (uint32_t *)r->d_buf, q.log2n, 1, q.root, q.mod, q.direction, q.flags, s);
+        if (rc) return rc;
+    }
+    for (ServerRequest *r : round) {
+        const stntt_request &q = r->req;
+        HIP_TRY(hipMemcpyAsync(q.host_out, r->d_buf, (size_t)elem_bytes(q.field) << q.log2n, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipEventRecord(ev, s));
+    return STNTT_OK;
+}
+
+static void server_loop(stntt_server *S) {
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        // 1. retire finished rounds
+        bool progressed = false;
+        for (auto it = S->rounds.begin(); it != S->rounds.end();) {
+            if (hipEventQuery(it->done_ev) != hipSuccess) {
+                ++it;
+                continue;
+            }
+            const auto now = Clock::now();
+            for (uint64_t id : it->ids) {
+                ServerRequest &r = S->reqs[id];
+                r.done = true;
+                r.latency_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
+                S->latencies.push_back(r.latency_ms);
+                S->used -= r.cost;
+                S->cache[r.d_bytes].push_back(r.d_buf);
+                r.d_buf = nullptr;
+                S->completed++;
+            }
+            S->t_last = now;
+            S->workers[it->worker].inflight--;
+            S->free_events.push_back(it->done_ev);
+            it = S->rounds.erase(it);
+            progressed = true;
+        }
+        if (progressed) S->cv_done.notify_all();
+        if (S->stop && S->pending.empty() && S->rounds.empty()) break;
+
+        // 2. admit a round onto a free worker
+        int w = -1;
+        for (size_t i = 0; i < S->workers.size(); ++i)
+            if (S->workers[i].inflight == 0) {
+                w = (int)i;
+                break;
+            }
+        std::vector<ServerRequest *> round;
+        if (w >= 0) {
+            while (!S->pending.empty() && round.size() < SERVER_MAX_ROUND) {
+                ServerRequest &r = S->reqs[S->pending.front()];
+                if (S->used + r.cost > S->budget) break;  // wait for memory to free up
+                void *buf = server_get_buffer(S, r.d_bytes);
+                if (!buf) break;
+                r.d_buf = buf;
+                S->used += r.cost;
+                S->peak_used = std::max(S->peak_used, S->used);
+                round.push_back(&r);
+                S->pending.pop_front();
+            }
+        }
+        if (!round.empty()) {
+            hipEvent_t ev;
+            if (!S->free_events.empty()) {
+                ev = S->free_events.back();
+                S->free_events.pop_back();
+            } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+                ev = nullptr;
+            }
+            S->workers[w].inflight++;
+            S->rounds_launched++;
+            for (ServerRequest *r : round) S->batched_requests += r->req.log2n <= SERVER_BATCH_MAX_LOG ? 1 : 0;
+            ServerRound rd{w, ev, {}};
+            for (ServerRequest *r : round) rd.ids.push_back(r->id);
+            lk.unlock();
+            int rc = ev ? server_launch_round(S, w, round, ev) : STNTT_EDEVICE;
+            lk.lock();
+            if (rc) {  // fail the round's requests; nothing of it is in flight
+                if (ev) (void)hipStreamSynchronize(S->workers[w].stream);
+                for (ServerRequest *r : round) {
+                    r->done = true;
+                    r->status = rc;
+                    S->used -= r->cost;
+                    S->cache[r->d_bytes].push_back(r->d_buf);
+                    r->d_buf = nullptr;
+                }
+                S->workers[w].inflight--;
+                if (ev) S->free_events.push_back(ev);
+                S->cv_done.notify_all();
+            } else {
+                S->rounds.push_back(std::move(rd));
+            }
+            continue;
+        }
+        if (!progressed) S->cv_work.wait_for(lk, std::chrono::microseconds(S->rounds.empty() ? 2000 : 20));
+    }
+}
+
 }  // namespace stntt
 
 using namespace stntt;
@@ -760,6 +944,163 @@ int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t ns
     int rc = batch_validate_split(log2n, batch_global_path(field, log2n), nsub, radix_logs);
     if (rc) return rc;
     ctx->batch_split[{field, log2n}] = {nsub, radix_logs};
+    return STNTT_OK;
+}
+
+int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out) {
+    if (!out) return fail(STNTT_EINVAL, "null out pointer");
+    if (workers < 1 || workers > 16) return fail(STNTT_EINVAL, "workers %u outside [1, 16]", workers);
+    const uint64_t reserve = workers * server_worker_reserve(SERVER_MAX_LOG);
+    if (hbm_budget_bytes <= reserve)
+        return fail(STNTT_EINVAL, "budget %llu does not cover the worker reservation %llu",
+                    (unsigned long long)hbm_budget_bytes, (unsigned long long)reserve);
+    auto S = std::make_unique<stntt_server>();
+    S->device = device;
+    S->budget = hbm_budget_bytes - reserve;
+    for (uint32_t i = 0; i < workers; ++i) {
+        ServerWorker W;
+        int rc = stntt_ctx_create(device, &W.ctx);
+        if (rc) return rc;
+        W.stream = W.ctx->stream;
+        // scratch for the largest served multi-pass transform, allocated now so
+        // that no later growth synchronises the device
+        if ((rc = ensure_buffer(&W.ctx->scratch, &W.ctx->scratch_bytes, 2 * (8ull << SERVER_MAX_LOG)))) return rc;
+        HIP_TRY(hipHostMalloc((void **)&W.h_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
+        HIP_TRY(hipMalloc(&W.d_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
+        HIP_TRY(hipMalloc(&W.d_counter, sizeof(uint32_t)));
+        HIP_TRY(hipMalloc(&W.d_scratch, (size_t)SERVER_BATCH_GRID * (8ull << SERVER_BATCH_MAX_LOG)));
+        S->workers.push_back(W);
+    }
+    stntt_server *raw = S.get();
+    S->thread = std::thread(server_loop, raw);
+    *out = S.release();
+    return STNTT_OK;
+}
+
+int stntt_server_destroy(stntt_server *srv) {
+    if (!srv) return STNTT_OK;
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    if (srv->thread.joinable()) srv->thread.join();
+    (void)hipSetDevice(srv->device);
+    for (auto &W : srv->workers) {
+        (void)hipStreamSynchronize(W.stream);
+        (void)hipHostFree(W.h_tasks);
+        (void)hipFree(W.d_tasks);
+        (void)hipFree(W.d_counter);
+        (void)hipFree(W.d_scratch);
+        stntt_ctx_destroy(W.ctx);
+    }
+    for (auto &kv : srv->cache)
+        for (void *p : kv.second) (void)hipFree(p);
+    for (auto &kv : srv->reqs)
+        if (kv.second.d_buf) (void)hipFree(kv.second.d_buf);
+    for (hipEvent_t e : srv->free_events) (void)hipEventDestroy(e);
+    delete srv;
+    return STNTT_OK;
+}
+
+int stntt_server_host_alloc(stntt_server *srv, size_t bytes, void **host_ptr) {
+    if (!srv || !host_ptr) return fail(STNTT_EINVAL, "null argument");
+    if (hipHostMalloc(host_ptr, bytes ? bytes : 1) != hipSuccess) {
+        *host_ptr = nullptr;
+        return fail(STNTT_ENOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+    }
+    return STNTT_OK;
+}
+
+int stntt_server_host_free(stntt_server *srv, void *host_ptr) {
+    if (!srv) return fail(STNTT_EINVAL, "null server");
+    HIP_TRY(hipHostFree(host_ptr));
+    return STNTT_OK;
+}
+
+int stntt_server_submit(stntt_server *srv, const stntt_request *req, uint64_t *id) {
+    if (!srv || !req || !id) return fail(STNTT_EINVAL, "null argument");
+    int rc = server_check_request(*req);
+    if (rc) return rc;
+    const uint64_t bytes = elem_bytes(req->field) << req->log2n;
+    if (req->log2n > SERVER_MAX_LOG) return fail(STNTT_EINVAL, "log2n %u above the served maximum %u", req->log2n,
+                                                 SERVER_MAX_LOG);
+    if (bytes > srv->budget) return fail(STNTT_EINVAL, "request of %llu bytes exceeds the HBM budget",
+                                         (unsigned long long)bytes);
+    {
+        std::lock_guard<std::mutex> g(srv->mu);
+        ServerRequest r;
+        r.id = srv->next_id++;
+        r.req = *req;
+        r.t_submit = Clock::now();
+        r.d_bytes = bytes;
+        r.cost = bytes;
+        if (!srv->any) {
+            srv->t_first = r.t_submit;
+            srv->any = true;
+        }
+        *id = r.id;
+        srv->reqs.emplace(r.id, r);
+        srv->pending.push_back(r.id);
+    }
+    srv->cv_work.notify_one();
+    return STNTT_OK;
+}
+
+int stntt_server_wait(stntt_server *srv, uint64_t id, double *latency_ms) {
+    if (!srv) return fail(STNTT_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end()) return fail(STNTT_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    srv->cv_done.wait(lk, [&] { return it->second.done; });
+    int status = it->second.status;
+    if (latency_ms) *latency_ms = it->second.latency_ms;
+    srv->reqs.erase(it);
+    if (status) return fail(status, "request %llu failed on the device", (unsigned long long)id);
+    return STNTT_OK;
+}
+
+int stntt_server_poll(stntt_server *srv, uint64_t id, int *done) {
+    if (!srv || !done) return fail(STNTT_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(srv->mu);
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end()) return fail(STNTT_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    *done = it->second.done ? 1 : 0;
+    return STNTT_OK;
+}
+
+int stntt_server_stats_get(stntt_server *srv, stntt_server_stats *out) {
+    if (!srv || !out) return fail(STNTT_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(srv->mu);
+    *out = stntt_server_stats{};
+    out->completed = srv->completed;
+    out->peak_bytes = srv->peak_used;
+    out->rounds = srv->rounds_launched;
+    out->batched = srv->batched_requests;
+    if (!srv->latencies.empty()) {
+        std::vector<double> v = srv->latencies;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double q) { return v[std::min(v.size() - 1, (size_t)(q * (v.size() - 1) + 0.5))]; };
+        out->p50_ms = pct(0.50);
+        out->p99_ms = pct(0.99);
+        out->max_ms = v.back();
+        double sum = 0;
+        for (double x : v) sum += x;
+        out->mean_ms = sum / v.size();
+        out->span_s = std::chrono::duration<double>(srv->t_last - srv->t_first).count();
+    }
+    return STNTT_OK;
+}
+
+int stntt_server_reset_stats(stntt_server *srv) {
+    if (!srv) return fail(STNTT_EINVAL, "null server");
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->latencies.clear();
+    srv->completed = 0;
+    srv->peak_used = srv->used;
+    srv->rounds_launched = 0;
+    srv->batched_requests = 0;
+    srv->any = false;
     return STNTT_OK;
 }
 

@@ -303,3 +303,75 @@ def set_batch_split(ctx: Context, field: str, log2n: int, radix_logs: list[int] 
     for i, lr in enumerate(radix_logs):
         packed |= lr << (4 * i)
     check(lib().stntt_set_batch_split(ctx.handle, f, log2n, len(radix_logs), packed))
+
+
+# --------------------------------------------------------------------------
+# serving
+# --------------------------------------------------------------------------
+class PinnedArray:
+    """numpy view of pinned host memory owned by a Server."""
+
+    def __init__(self, server: "Server", count: int, dtype):
+        self.server = server
+        nbytes = max(int(count) * np.dtype(dtype).itemsize, 1)
+        p = ctypes.c_void_p()
+        check(lib().stntt_server_host_alloc(server.handle, nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        buf = (ctypes.c_char * nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(count))
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().stntt_server_host_free(self.server.handle, self.ptr)
+            self.ptr = None
+            self.array = None
+
+
+class Server:
+    """Continuous-batching transform server (include/stntt.h, serving)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, workers: int = 4):
+        h = ctypes.c_void_p()
+        check(lib().stntt_server_create(device, hbm_budget_bytes, workers, ctypes.byref(h)))
+        self.handle = h
+
+    def pinned(self, count: int, dtype) -> PinnedArray:
+        return PinnedArray(self, count, dtype)
+
+    def submit(self, field: str, log2n: int, data: PinnedArray, inverse: bool = False, root: int = 0,
+               mod: int = 0, out: PinnedArray | None = None) -> int:
+        req = _lib.Request(_lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32,
+                           _lib.INVERSE if inverse else _lib.FORWARD, log2n, 0, root, mod, data.ptr,
+                           (out or data).ptr)
+        rid = ctypes.c_uint64()
+        check(lib().stntt_server_submit(self.handle, ctypes.byref(req), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int) -> float:
+        ms = ctypes.c_double()
+        check(lib().stntt_server_wait(self.handle, rid, ctypes.byref(ms)))
+        return ms.value
+
+    def poll(self, rid: int) -> bool:
+        d = ctypes.c_int()
+        check(lib().stntt_server_poll(self.handle, rid, ctypes.byref(d)))
+        return bool(d.value)
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        check(lib().stntt_server_stats_get(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def reset_stats(self) -> None:
+        check(lib().stntt_server_reset_stats(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().stntt_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
