@@ -212,6 +212,7 @@ def bench_serving(S, count: int, workers: int) -> dict:
 
     out = {"requests": count, "workers": workers, "budget_bytes": 64 << 30,
            "plain_sequential_capacity_tps": round(seq_capacity, 1), "rates": []}
+    replay(4.0 * seq_capacity, True)  # warm the server's device-buffer cache and plans
     for frac in (0.8, 2.0):
         rate = frac * seq_capacity
         lat_plain, span_plain = replay(rate, False)
