@@ -130,6 +130,10 @@ int stntt_batch_destroy(stntt_batch *b);
    4 bits per sub-step (first in the low bits), as chosen by the autotuner.
    nsub = 0 restores the built-in split. */
 int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs);
+/* Tuning: stages per tile pass of the multi-pass path for (field, log2n),
+   one log2 radix per pass (first pass first), as chosen by the autotuner;
+   npasses = 0 restores the built-in split.  Each pass covers 4..9 stages. */
+int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr);
 
 /* ---- serving: continuous batching under an HBM budget -------------------
    No counterpart in the reference, whose drivers run one synchronous

@@ -1,0 +1,173 @@
+"""Autotuner: per (field, log2 n) radix splits, measured on the GPU.
+
+Two choices are tuned and installed into a context through the C ABI:
+  * the stages per tile pass of the multi-pass path (stntt_set_pass_split),
+    timed on a single resident transform of that size;
+  * the radix split of the batched kernel's sub-steps (stntt_set_batch_split),
+    timed on a batch of same-shape transforms in one persistent launch.
+The winners are written to tuning/gfx950.json next to this file and applied
+to every new Context (Context(tuned=False) skips it).
+
+    python -m stntt_amd.autotune            # on an MI355X
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning", "gfx950.json")
+GL_P = 0xFFFFFFFF00000001
+FIELDS = ("u32", "goldilocks")
+
+
+def apply_table(ctx, path: str = TABLE) -> int:
+    """Install a tuning table into ctx; returns the number of entries applied."""
+    from .ntt import set_batch_split, set_pass_split
+    if not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        table = json.load(f)
+    k = 0
+    for field in FIELDS:
+        for L, logr in table.get("pass_split", {}).get(field, {}).items():
+            set_pass_split(ctx, field, int(L), logr)
+            k += 1
+        for L, logs in table.get("batch_split", {}).get(field, {}).items():
+            set_batch_split(ctx, field, int(L), logs)
+            k += 1
+    return k
+
+
+def pass_candidates(L: int, logc: int, lo: int = 4, hi: int = 9) -> list[list[int]]:
+    out = []
+    for np_ in (2, 3, 4):
+        for parts in itertools.product(range(lo, hi + 1), repeat=np_):
+            if sum(parts) != L:
+                continue
+            if any(L - p < logc for p in parts):
+                continue
+            if any(sum(parts[:i]) < logc for i in range(1, np_)):
+                continue
+            out.append(list(parts))
+    # fewest passes first, then the most even splits
+    out.sort(key=lambda p: (len(p), max(p) - min(p)))
+    return out[:8]
+
+
+def batch_candidates(L: int, streamed: bool) -> list[list[int]]:
+    """Radix splits from {4, 8, 16} sub-steps (2^2..2^4), fewest sub-steps
+    first, each multiset in descending and ascending order; streamed sizes
+    need an even count (the ping-pong must end in the destination)."""
+    nmin = (L + 3) // 4
+    out = []
+    for ns in range(nmin, nmin + 3):
+        if streamed and ns % 2:
+            continue
+        for parts in itertools.combinations_with_replacement((4, 3, 2), ns):
+            if sum(parts) != L:
+                continue
+            desc = sorted(parts, reverse=True)
+            for order in (desc, desc[::-1]):
+                if order not in out:
+                    out.append(list(order))
+    if not out:  # small sizes: fall back to radix-2 parts
+        ns = L if not streamed or L % 2 == 0 else L + 1
+        out.append([1] * ns if sum([1] * ns) == L else [1] * L)
+    return out[:6]
+
+
+def _time(fn, sync, reps: int) -> float:
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True) -> dict:
+    import stntt_amd as S
+    from .ntt import set_batch_split, set_pass_split
+    ctx = S.Context(0, tuned=False)
+    rng = np.random.default_rng(1)
+    table = {"device": "gfx950", "pass_split": {f: {} for f in FIELDS}, "batch_split": {f: {} for f in FIELDS},
+             "measured_us": {"pass": {f: {} for f in FIELDS}, "batch": {f: {} for f in FIELDS}}}
+    for field in FIELDS:
+        logc = 4 if field == "goldilocks" else 5
+        for L in logs_pass:
+            n = 1 << L
+            if field == "u32":
+                root, mod = S.find_params(n, 11)
+                x = rng.integers(0, mod, size=n, dtype=np.uint32)
+            else:
+                x = rng.integers(0, GL_P, size=n, dtype=np.uint64)
+            buf = ctx.upload(x)
+            best, best_t, times = None, 1e9, {}
+            for cand in pass_candidates(L, logc):
+                set_pass_split(ctx, field, L, cand)
+                if field == "u32":
+                    t = _time(lambda: ctx.ntt_u32(buf, L, 1, root, mod), ctx.sync, 20)
+                else:
+                    t = _time(lambda: ctx.ntt_goldilocks(buf, L, 1), ctx.sync, 20)
+                times[str(cand)] = round(t * 1e6, 2)
+                if t < best_t:
+                    best, best_t = cand, t
+            set_pass_split(ctx, field, L, best)
+            table["pass_split"][field][str(L)] = best
+            table["measured_us"]["pass"][field][str(L)] = times
+            if verbose:
+                print(f"[tune] pass {field} 2^{L}: {best} {best_t * 1e6:.1f} us", flush=True)
+            buf.free()
+        for L in logs_batch:
+            n = 1 << L
+            count = max(64, (1 << 22) >> L)
+            w = 8 if field == "goldilocks" else 4
+            streamed = (w << L) > 32768
+            if field == "u32":
+                root, mod = S.find_params(n, 11)
+            else:
+                root, mod = 0, GL_P
+            bufs = []
+            for _ in range(count):
+                x = rng.integers(0, mod if field == "u32" else GL_P, size=n,
+                                 dtype=np.uint32 if field == "u32" else np.uint64)
+                bufs.append(ctx.upload(x))
+            best, best_t, times = None, 1e9, {}
+            for cand in batch_candidates(L, streamed):
+                set_batch_split(ctx, field, L, cand)
+                b = S.Batch(ctx, [(field, bool(i & 1), L, bb, root, mod) for i, bb in enumerate(bufs)])
+                t = _time(b.run, ctx.sync, 5)
+                b.close()
+                times[str(cand)] = round(t * 1e6, 2)
+                if t < best_t:
+                    best, best_t = cand, t
+            set_batch_split(ctx, field, L, best)
+            table["batch_split"][field][str(L)] = best
+            table["measured_us"]["batch"][field][str(L)] = times
+            if verbose:
+                print(f"[tune] batch {field} 2^{L} x{count}: {best} {best_t * 1e6:.1f} us", flush=True)
+            for bb in bufs:
+                bb.free()
+    ctx.close()
+    return table
+
+
+def main() -> int:
+    out = sys.argv[1] if len(sys.argv) > 1 else TABLE
+    table = tune()
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(table, f, indent=1)
+    print("wrote", out)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -124,6 +124,8 @@ struct stntt_ctx {
     int prof_count = 0;  // events recorded by the last profiled transform
     // autotuned sub-step splits of the batched kernel: (field, log2n) -> (nsub, radix_logs)
     std::map<std::pair<int, uint32_t>, std::pair<uint32_t, uint32_t>> batch_split;
+    // autotuned tile-pass splits: (field, log2n) -> stages per pass
+    std::map<std::pair<int, uint32_t>, std::vector<int>> pass_split;
     int num_cus = 0;
 };
 
@@ -268,6 +270,13 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     if (batches == 0) return STNTT_OK;
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
     Schedule sch = make_schedule<FP>(L, batches);
+    if (!sch.single) {
+        auto it = ctx->pass_split.find({plan.key.field, L});
+        if (it != ctx->pass_split.end()) {
+            sch.npasses = (int)it->second.size();
+            for (int i = 0; i < sch.npasses; ++i) sch.logr[i] = it->second[i];
+        }
+    }
     const bool inv = plan.key.dir == STNTT_INVERSE;
     // intermediates alternate between two scratch halves so that the last
     // pass always lands in the caller's buffer (no copy-back)
@@ -1101,6 +1110,32 @@ int stntt_server_reset_stats(stntt_server *srv) {
     srv->rounds_launched = 0;
     srv->batched_requests = 0;
     srv->any = false;
+    return STNTT_OK;
+}
+
+int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (field != STNTT_FIELD_U32 && field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");
+    if (npasses == 0) {
+        ctx->pass_split.erase({field, log2n});
+        return STNTT_OK;
+    }
+    if (npasses < 2 || npasses > 4 || !logr) return fail(STNTT_EINVAL, "pass count %u outside [2, 4]", npasses);
+    const int logc = field == STNTT_FIELD_GOLDILOCKS ? tile_logc<FieldGL>() : tile_logc<FieldU32>();
+    uint32_t sum = 0;
+    std::vector<int> v;
+    for (uint32_t i = 0; i < npasses; ++i) {
+        if (logr[i] < (uint32_t)MIN_LOGR_TILE || logr[i] > (uint32_t)MAX_LOGR_TILE)
+            return fail(STNTT_EINVAL, "pass radix 2^%u outside [2^%d, 2^%d]", logr[i], MIN_LOGR_TILE, MAX_LOGR_TILE);
+        // a pass needs n / R >= C columns, and a later pass J >= C
+        if (i > 0 && sum < (uint32_t)logc) return fail(STNTT_EINVAL, "pass %u starts with J < %d columns", i, 1 << logc);
+        sum += logr[i];
+        v.push_back((int)logr[i]);
+    }
+    if (sum != log2n) return fail(STNTT_EINVAL, "pass radices cover 2^%u, not 2^%u", sum, log2n);
+    for (uint32_t i = 0; i < npasses; ++i)
+        if (log2n - logr[i] < (uint32_t)logc) return fail(STNTT_EINVAL, "pass %u has fewer than %d columns", i, 1 << logc);
+    ctx->pass_split[{field, log2n}] = v;
     return STNTT_OK;
 }
 

@@ -47,11 +47,14 @@ def _log2_exact(n: int) -> int:
 class Context:
     """One device: its stream, resident twiddle plans and scratch pool."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuned: bool = True):
         h = ctypes.c_void_p()
         check(lib().stntt_ctx_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        if tuned:
+            from .autotune import apply_table
+            apply_table(self)
 
     def close(self) -> None:
         if self.handle:
@@ -291,6 +294,13 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def set_pass_split(ctx: Context, field: str, log2n: int, logr: list[int] | None) -> None:
+    """Install an autotuned stages-per-pass split for the multi-pass path."""
+    f = _lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32
+    arr = (ctypes.c_uint32 * max(len(logr or []), 1))(*(logr or []))
+    check(lib().stntt_set_pass_split(ctx.handle, f, log2n, len(logr or []), arr))
 
 
 def set_batch_split(ctx: Context, field: str, log2n: int, radix_logs: list[int] | None) -> None:

@@ -219,3 +219,32 @@ def test_errors(ctx):
         S.ntt_stockham(np.zeros(16, dtype=np.int32), 16, 2, 17, ctx=ctx)  # 2 is not of order 16 mod 17
     with pytest.raises(ValueError):
         S.ntt_stockham(np.zeros(16, dtype=np.int32), 16, 3, 2**31 + 11, ctx=ctx)  # modulus too large
+
+
+@pytest.mark.parametrize("field,L,split", [("goldilocks", 24, [9, 8, 7]), ("goldilocks", 16, [6, 5, 5]),
+                                           ("u32", 20, [6, 7, 7]), ("u32", 22, [9, 9, 4]),
+                                           ("goldilocks", 20, [5, 5, 5, 5])])
+def test_tuned_pass_splits_same_result(ctx, field, L, split):
+    """Any valid stages-per-pass split gives the same bits."""
+    n = 1 << L
+    S.set_pass_split(ctx, field, L, split)
+    try:
+        if field == "goldilocks":
+            x = gl_input(3000 + L, n)
+            assert np.array_equal(run_gl(ctx, x, L, 1), O.stockham_gl(x, L))
+            assert np.array_equal(run_gl(ctx, x, L, 1, inverse=True), O.inverse_gl(x, L))
+        else:
+            root, mod = S.find_params(n, 11)
+            x = u32_input(3000 + L, n, mod)
+            assert np.array_equal(run_u32(ctx, x, L, 1, root, mod), O.stockham_u32(x, n, root, mod))
+    finally:
+        S.set_pass_split(ctx, field, L, None)
+
+
+def test_pass_split_validation(ctx):
+    with pytest.raises(ValueError):
+        S.set_pass_split(ctx, "goldilocks", 24, [8, 8, 7])     # covers 23 stages
+    with pytest.raises(ValueError):
+        S.set_pass_split(ctx, "goldilocks", 24, [12, 12])      # pass above 2^9
+    with pytest.raises(ValueError):
+        S.set_pass_split(ctx, "u32", 12, [4, 4, 4])            # J = 16 < 32 columns

@@ -268,3 +268,27 @@ def test_fixtures_discriminate_plausible_mistakes(ref_u32):
     assert not np.array_equal(O.stockham_u32(x, n, root, mod, 2)[:n][perm], want[:n])
     swapped = np.concatenate([x[n:], x[:n]])
     assert not np.array_equal(O.stockham_u32(swapped, n, root, mod, 2), want)
+
+
+@pytest.mark.parametrize("L", [1, 6, 11])
+def test_sampled_dft_points_agree_with_full_transforms(L):
+    n = 1 << L
+    ks = np.array(sorted({0, 1, n // 2, n - 1, (3 * n) // 7}), dtype=np.uint64)
+    x = splitmix64(70 + L, n)
+    assert np.array_equal(O.dft_points_gl(x, L, ks), O.stockham_gl(x, L)[ks])
+    assert np.array_equal(O.dft_points_gl(x, L, ks, inverse=True), O.inverse_gl(x, L)[ks])
+    root, mod = O.find_params(n, 11)
+    y = (x % np.uint64(mod)).astype(np.uint32)
+    assert np.array_equal(O.dft_points_u32(y, n, root, mod, ks), O.stockham_u32(y, n, root, mod)[ks])
+    assert np.array_equal(O.dft_points_u32(y, n, root, mod, ks, inverse=True), O.inverse_u32(y, n, root, mod)[ks])
+
+
+@pytest.mark.parametrize("L", [1, 5, 10])
+def test_twiddle_tables_are_root_powers(L):
+    n = 1 << L
+    root, mod = O.find_params(n, 11)
+    assert [int(v) for v in O.twiddles_u32(n, root, mod)] == [pow(root, i, mod) for i in range(n // 2)]
+    w = pow(7, (P - 1) >> L, P)
+    assert [int(v) for v in O.twiddles_gl(L)] == [pow(w, i, P) for i in range(n // 2)]
+    wi = pow(w, P - 2, P)
+    assert [int(v) for v in O.twiddles_gl(L, inverse=True)] == [pow(wi, i, P) for i in range(n // 2)]
