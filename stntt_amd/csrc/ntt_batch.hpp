@@ -11,7 +11,7 @@
 //     smaller) Stockham sub-steps run in registers between LDS exchanges;
 //   * larger n: the same sub-steps stream through global memory, ping-
 //     ponging between the destination and a scratch slice private to the
-//     workgroup, with a workgroup barrier and a device-scope fence between
+//     workgroup, with a workgroup barrier and workgroup-scope fences between
 //     sub-steps.
 // No workgroup ever waits for another, so the kernel needs no inter-
 // workgroup synchronisation and every wave reaches the exit when the counter
@@ -177,10 +177,14 @@ __device__ void batch_run_task(const BatchTask &t, typename FP::T *scratch, unsi
                 case 2: batch_substep_global<FP, 2, DIR>(in, out, L, logj, t, rt, fl, last); break;
                 default: batch_substep_global<FP, 1, DIR>(in, out, L, logj, t, rt, fl, last); break;
             }
-            // make this workgroup's global writes visible to its own next reads
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            // Make this workgroup's global writes visible to its own next
+            // reads.  Workgroup scope suffices: no other workgroup touches
+            // this transform's buffers during the launch, and the waves of a
+            // workgroup share one CU's caches.  (Agent scope would write back
+            // and invalidate the whole L2 of the XCD at every sub-step.)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             in = out;
             logj += lr;
         }

@@ -79,6 +79,8 @@ struct FieldGL {
     };
     // runtime roots are not needed: omega_r^j is a power of two
     struct Roots {};
+    // a twiddle times a twiddle is a twiddle: running products are cheap
+    static constexpr bool RUNNING_TWIDDLES = true;
 
     static __device__ __forceinline__ T add(T a, T b, const Rt &) { return gl_add(a, b); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &) { return gl_sub(a, b); }
@@ -116,6 +118,8 @@ struct FieldU32 {
     struct Roots {
         U32Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
     };
+    // a product of twiddles needs a fresh Shoup companion (a division)
+    static constexpr bool RUNNING_TWIDDLES = false;
 
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32_sub(a, b, rt.p); }

@@ -26,6 +26,18 @@
 
 namespace stntt {
 
+// element at a 32-bit byte offset from a uniform base: lets the compiler use
+// the scalar-base + 32-bit vector-offset addressing form (offsets here stay
+// below 2^31: n <= 2^27 words of at most 8 bytes)
+template <class T>
+__device__ __forceinline__ const T &at_off(const T *base, uint32_t elem) {
+    return *(const T *)((const char *)base + elem * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T &at_off(T *base, uint32_t elem) {
+    return *(T *)((char *)base + elem * (uint32_t)sizeof(T));
+}
+
 template <class FP>
 constexpr int tile_logc() {
     return std::is_same_v<FP, FieldGL> ? 4 : 5;  // 128-byte rows
@@ -41,61 +53,95 @@ constexpr int tile_threads() {
     return (1 << (LOGR - tile_lr1<LOGR>())) << tile_logc<FP>();  // r2 * C
 }
 
+// One tile of a pass, computed by NT cooperating threads (NT need not equal
+// the natural (R/r1) C: threads loop over groups and idle ones predicate
+// off).  `block` is the tile's linear index over the whole batch.  lds holds
+// R * C words, twl R twiddles.
 // J1: first pass of a schedule (J = 1): outputs of a column are contiguous
 // (y[R t + k]); otherwise J >= C and a row of C outputs is contiguous.
-template <class FP, int LOGR, int DIR, bool J1>
-__global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(PassArgs<FP> a) {
+template <class FP, int LOGR, int DIR, bool J1, int NT>
+__device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block, typename FP::T *lds,
+                                          typename FP::Tw *twl) {
     using T = typename FP::T;
+    using Tw = typename FP::Tw;
     constexpr int LOGC = tile_logc<FP>();
     constexpr int C = 1 << LOGC;
     constexpr int R = 1 << LOGR;
     constexpr int LR1 = tile_lr1<LOGR>(), LR2 = LOGR - LR1;
     constexpr int r1 = 1 << LR1, r2 = 1 << LR2;
-    constexpr int NT = tile_threads<FP, LOGR>();
-    constexpr int G2 = r1 / r2;  // sub-step-2 groups per thread (1 or 2)
-    static_assert(NT == (R / r1) * C, "one sub-step-1 group per thread");
-
-    __shared__ T lds[R * C];
-    __shared__ typename FP::Tw twl[J1 ? 1 : R];  // shared pass twiddles w^(s J k), k < R
+    constexpr int NG1 = (R / r1) * C, NG2 = r1 * C;  // groups of the two sub-steps
+    constexpr int G1 = (NG1 + NT - 1) / NT, G2 = (NG2 + NT - 1) / NT;
+    constexpr int TWL = (R + NT - 1) / NT;  // pass twiddles built per thread
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
     const uint32_t tpt = a.tiles_per_transform;
-    const uint32_t b = blockIdx.x / tpt;
-    const uint32_t t0 = (blockIdx.x % tpt) << LOGC;  // first column t of the tile
+    const uint32_t b = block / tpt;
+    const uint32_t t0 = (block % tpt) << LOGC;  // first column t of the tile
     const uint64_t base = (uint64_t)b << logn;
     const uint32_t stride_in = 1u << (logn - LOGR);  // n / R
+    const uint32_t log_b = logn - a.log_rmax;        // w^e = hi[e >> log_b] lo[e & bmask]
+    const uint32_t bmask = (1u << log_b) - 1;
     const T *in = a.in + base;
     T *out = a.out + base;
-
-    // ---- pass twiddles shared by the tile (all columns have the same s) ----
+    // all columns of the tile share s, hence the pass twiddles w^(s J k)
     const uint32_t sJ = J1 ? 0u : ((t0 >> a.log_j) << a.log_j);
-    if constexpr (!J1) {
-        if (sJ != 0) {
-            for (uint32_t k = tid; k < (uint32_t)R; k += NT) {
-                const uint32_t e = sJ * k;  // < n
-                const uint32_t log_b = logn - a.log_rmax;
-                twl[k] = FP::twiddle(a.hi[e >> log_b], a.lo[e & ((1u << log_b) - 1)], a.rt);
-            }
-        }
-    }
 
     // ---- sub-step 1: global -> registers -> DFT_r1 -> twiddle -> LDS -------
-    {
-        const uint32_t c = tid & (C - 1), g = tid >> LOGC;
+    // Every global load of a group is issued before any of them is used (the
+    // data rows, the inner twiddles w_R^(g k) and, with the first group, the
+    // factors of the pass twiddles), so a tile waits for memory once rather
+    // than once per dependent load.  Row bases are uniform pointers; a lane
+    // adds one 32-bit element offset.
+    const uint64_t mstep = (uint64_t)(R / r1) * stride_in;
+    const auto roots = FP::template load_roots<LR1, DIR>(a.hi, a.log_rmax);
+#pragma unroll
+    for (int gi = 0; gi < G1; ++gi) {
+        const uint32_t q = tid + gi * NT;
+        const bool act = NG1 % NT == 0 || q < (uint32_t)NG1;
+        const uint32_t c = q & (C - 1), g = q >> LOGC;
+        const uint32_t lane = t0 + c + g * stride_in;
+        const uint32_t tstep = g << (a.log_rmax - LOGR);  // w_R^(g k) = hi[k tstep]
         T v[r1];
-        const T *src = in + t0 + c + (uint64_t)g * stride_in;
+        Tw w1[r1];
+        if (act) {
 #pragma unroll
-        for (int m = 0; m < r1; ++m) v[m] = FP::load_in(src[(uint64_t)m * (R / r1) * stride_in], a.rt, a.flags);
-        const auto roots = FP::template load_roots<LR1, DIR>(a.hi, a.log_rmax);
-        dft_regs<FP, LR1, DIR>(v, roots, a.rt);
+            for (int m = 0; m < r1; ++m) v[m] = at_off(in + (uint64_t)m * mstep, lane);
 #pragma unroll
-        for (int k = 0; k < r1; ++k) {
-            T x = v[bitrev_c(k, LR1)];
-            // hi[0] = 1: lanes with g = 0 multiply by one rather than diverge
-            if (k != 0) x = FP::mul(x, a.hi[(g * k) << (a.log_rmax - LOGR)], a.rt);
-            const uint32_t row = r1 * g + k;
-            lds[row * C + (c ^ (row & (C - 1)))] = x;
+            for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
+        }
+        // keep the scheduler from pulling a dependent use (and its wait)
+        // in between the loads
+        __builtin_amdgcn_sched_barrier(0);
+        if constexpr (!J1) {
+            if (gi == 0 && sJ != 0) {
+                Tw fh[TWL], fl[TWL];
+#pragma unroll
+                for (int i = 0; i < TWL; ++i) {
+                    const uint32_t k = tid + i * NT, e = sJ * k;  // e < n
+                    if (R % NT == 0 || k < (uint32_t)R) {
+                        fh[i] = at_off(a.hi, e >> log_b);
+                        fl[i] = at_off(a.lo, e & bmask);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < TWL; ++i) {
+                    const uint32_t k = tid + i * NT;
+                    if (R % NT == 0 || k < (uint32_t)R) twl[k] = FP::twiddle(fh[i], fl[i], a.rt);
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < r1; ++m) v[m] = FP::load_in(v[m], a.rt, a.flags);
+            dft_regs<FP, LR1, DIR>(v, roots, a.rt);
+#pragma unroll
+            for (int k = 0; k < r1; ++k) {
+                T x = v[bitrev_c(k, LR1)];
+                if (k != 0) x = FP::mul(x, w1[k], a.rt);
+                const uint32_t row = r1 * g + k;
+                lds[row * C + (c ^ (row & (C - 1)))] = x;
+            }
         }
     }
     __syncthreads();
@@ -105,6 +151,7 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(Pa
 #pragma unroll
     for (int gi = 0; gi < G2; ++gi) {
         const uint32_t q = tid + gi * NT;
+        if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) continue;
         uint32_t c, g2;
         if constexpr (J1) {  // outputs of a column are contiguous: lanes walk g2
             g2 = q & (r1 - 1);
@@ -113,6 +160,31 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(Pa
             c = q & (C - 1);
             g2 = q >> LOGC;
         }
+        // J = 1: output k = k2 r1 + g2 of column s = t0 + c takes w^(s k).
+        // Its factors are loaded before the LDS reads so their latency
+        // overlaps them and the DFT.  Goldilocks walks a running product
+        // w^(s g2) (w^(s r1))^k2 (exact, so the same words as a lookup, from
+        // four loads per thread); the 32-bit field multiplies by hi and lo
+        // (a running product would need a new Shoup companion per step).
+        constexpr int NTW = !J1 ? 1 : (FP::RUNNING_TWIDDLES ? 2 : r2);
+        Tw th[NTW], tl[NTW];
+        if constexpr (J1) {
+            const uint32_t s = t0 + c;
+            if constexpr (FP::RUNNING_TWIDDLES) {
+                const uint32_t e0 = s * g2, es = s * r1;  // < n
+                th[0] = at_off(a.hi, e0 >> log_b);
+                tl[0] = at_off(a.lo, e0 & bmask);
+                th[1] = at_off(a.hi, es >> log_b);
+                tl[1] = at_off(a.lo, es & bmask);
+            } else {
+                uint32_t e = s * g2;
+#pragma unroll
+                for (int k2 = 0; k2 < r2; ++k2, e += s * r1) {
+                    th[k2] = at_off(a.hi, e >> log_b);
+                    tl[k2] = at_off(a.lo, e & bmask);
+                }
+            }
+        }
         T v[r2];
 #pragma unroll
         for (int m = 0; m < r2; ++m) {
@@ -120,24 +192,43 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(Pa
             v[m] = lds[row * C + (c ^ (row & (C - 1)))];
         }
         dft_regs<FP, LR2, DIR>(v, roots2, a.rt);
+        [[maybe_unused]] Tw cur, step;
+        if constexpr (J1 && FP::RUNNING_TWIDDLES) {
+            cur = FP::mul(th[0], tl[0], a.rt);
+            step = FP::mul(th[1], tl[1], a.rt);
+        }
+        // stores: a uniform base per k2 plus one 32-bit lane offset
+        const uint32_t lane_out = J1 ? (c * R + g2) : ((g2 << a.log_j) + c);
 #pragma unroll
         for (int k2 = 0; k2 < r2; ++k2) {
             const uint32_t k = k2 * r1 + g2;
             T x = v[bitrev_c(k2, LR2)];
-            // uniform code: hi[0] = lo[0] = twl[0] = 1
             if constexpr (J1) {
-                const uint32_t e = (t0 + c) * k;  // s = t0 + c, J = 1
-                const uint32_t log_b = logn - a.log_rmax;
-                x = FP::mul(x, a.hi[e >> log_b], a.rt);
-                if (log_b) x = FP::mul(x, a.lo[e & ((1u << log_b) - 1)], a.rt);
+                if constexpr (FP::RUNNING_TWIDDLES) {
+                    x = FP::mul(x, cur, a.rt);
+                    if (k2 + 1 < r2) cur = FP::mul(cur, step, a.rt);
+                } else {
+                    x = FP::mul(x, th[k2], a.rt);
+                    if (log_b) x = FP::mul(x, tl[k2], a.rt);
+                }
             } else {
-                if (sJ != 0) x = FP::mul(x, twl[k], a.rt);
+                if (sJ != 0) x = FP::mul(x, twl[k], a.rt);  // uniform branch
             }
             if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
-            if constexpr (J1) out[(uint64_t)(t0 + c) * R + k] = x;
-            else out[(uint64_t)(sJ * R) + ((uint64_t)k << a.log_j) + (t0 - sJ) + c] = x;
+            T *orow = J1 ? out + (uint64_t)t0 * R + k2 * r1
+                         : out + (uint64_t)sJ * R + (t0 - sJ) + ((uint64_t)(k2 * r1) << a.log_j);
+            at_off(orow, lane_out) = x;
         }
     }
+}
+
+template <class FP, int LOGR, int DIR, bool J1>
+__global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(PassArgs<FP> a) {
+    constexpr int C = 1 << tile_logc<FP>();
+    constexpr int R = 1 << LOGR;
+    __shared__ typename FP::T lds[R * C];
+    __shared__ typename FP::Tw twl[J1 ? 1 : R];
+    tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>()>(a, blockIdx.x, lds, twl);
 }
 
 }  // namespace stntt
