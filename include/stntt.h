@@ -44,8 +44,16 @@ typedef enum stntt_status {
 typedef enum stntt_field { STNTT_FIELD_U32 = 0, STNTT_FIELD_GOLDILOCKS = 1 } stntt_field;
 typedef enum stntt_direction { STNTT_FORWARD = 0, STNTT_INVERSE = 1 } stntt_direction;
 
-/* flags for the 32-bit field */
-#define STNTT_SIGNED_INPUT 1u /* inputs are int32 (negative values allowed, as in the reference) */
+/* transform flags */
+#define STNTT_SIGNED_INPUT 1u /* 32-bit field: inputs are int32 (negative values allowed, as in the reference) */
+/* Element order.  Default: natural in, natural out (nttStockham).
+   STNTT_BITREV_OUT: natural in, bit-reversed out -- output word i holds
+   X[bitrev(i)], the order of nttCt (src/ntt/ntt_cpu.cpp:11-42).
+   STNTT_BITREV_IN: bit-reversed in, natural out -- the order inttGs takes
+   (src/ntt/ntt_cpu.cpp:130-164).  Either direction accepts either flag (or
+   both). */
+#define STNTT_BITREV_IN 2u
+#define STNTT_BITREV_OUT 4u
 
 typedef struct stntt_ctx stntt_ctx;
 typedef void *stntt_stream; /* a hipStream_t */
@@ -83,6 +91,28 @@ int stntt_ntt_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t bat
                   uint32_t mod, int direction, uint32_t flags, stntt_stream stream);
 int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
                          stntt_stream stream);
+/* stntt_ntt_goldilocks with transform flags (STNTT_BITREV_IN / _OUT) */
+int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                            uint32_t flags, stntt_stream stream);
+
+/* ---- pointwise products and polynomial multiplication ------------------ */
+/* a[i] = a[i] * b[i] mod p for i < count, a and b canonical (< p):
+   elemWiseMul (src/ntt/ntt_util.h:28-33) on device buffers. */
+int stntt_pointwise_u32(stntt_ctx *ctx, uint32_t *d_a, const uint32_t *d_b, uint64_t count, uint32_t mod,
+                        stntt_stream stream);
+int stntt_pointwise_goldilocks(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_b, uint64_t count,
+                               stntt_stream stream);
+/* Cyclic (positive-wrapped) convolution of batches of length-2^log2n
+   vectors, as pwcNtt (src/ntt/ntt_cpu.cpp:167-174): afterwards a holds
+   a (*) b and b holds nttCt(b), its forward transform in bit-reversed order.
+   Three transform launches sequences: forward(a) and forward(b) with
+   bit-reversed output, then the inverse with bit-reversed input whose first
+   pass multiplies by b as it loads a (the pointwise product is fused).
+   flags: STNTT_SIGNED_INPUT (32-bit field) for int32 inputs. */
+int stntt_polymul_u32(stntt_ctx *ctx, uint32_t *d_a, uint32_t *d_b, uint32_t log2n, uint64_t batches,
+                      uint32_t root, uint32_t mod, uint32_t flags, stntt_stream stream);
+int stntt_polymul_goldilocks(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint32_t log2n, uint64_t batches,
+                             stntt_stream stream);
 
 /* ---- transforms on host buffers (synchronous, autoNtt-shaped) ---------- */
 /* Copies host_vec[0 .. n*batches) to the device, transforms it, copies it
@@ -96,6 +126,15 @@ int stntt_auto_ntt_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint3
                        uint32_t root, uint32_t mod, int direction, float *kernel_ms);
 int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_len, uint32_t n,
                               uint32_t batches, int direction, float *kernel_ms);
+/* the same with transform flags: STNTT_BITREV_OUT gives nttCt
+   (src/ntt/ntt_cpu.cpp:11-42), STNTT_INVERSE with STNTT_BITREV_IN gives
+   inttGs (src/ntt/ntt_cpu.cpp:130-164) */
+int stntt_auto_ntt_i32_ex(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                          uint32_t root, uint32_t mod, int direction, uint32_t flags, float *kernel_ms);
+/* pwcNtt (src/ntt/ntt_cpu.cpp:167-174) on host int vectors of length n:
+   host_a becomes a (*) b, host_b becomes nttCt(b); *kernel_ms as above */
+int stntt_auto_polymul_i32(stntt_ctx *ctx, int32_t *host_a, int32_t *host_b, uint32_t n, uint32_t root,
+                           uint32_t mod, float *kernel_ms);
 
 /* ---- batched mixed transforms -------------------------------------------
    Thousands of independent transforms of mixed size, field and direction in

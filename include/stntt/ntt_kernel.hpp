@@ -6,6 +6,11 @@
 //   cuda::ntt::autoNtt<n>(args)   src/cuda/ntt/ntt_kernel.cuh:152-201
 //   findParams(size, min_mod)     src/ntt/ntt_util.cpp:10-14
 //   nttStockham(span, size, root, mod, batches)   src/ntt/ntt_cpu.cpp:87-90
+//   nttCt(vec, root, mod)         src/ntt/ntt_cpu.cpp:11-42
+//   inttGs(vec, root, mod)        src/ntt/ntt_cpu.cpp:130-164
+//   pwcNtt(vec0, vec1, root, mod) src/ntt/ntt_cpu.cpp:167-174
+//   pwc_convolution(s0, s1)       src/ntt/ntt_cpu.cpp:180-192
+//   elemWiseMul(s0, s1, mod)      src/ntt/ntt_util.h:28-33
 // A caller switches by including this header instead of
 // "cuda/ntt/ntt_kernel.cuh" / "ntt/ntt_cpu.h" and linking libstntt.so.
 // Errors: std::invalid_argument (bad size or parameters), std::out_of_range
@@ -19,6 +24,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "../stntt.h"
 
@@ -117,6 +123,54 @@ inline void inttStockham(std::span<int> vec, int size, int root, int mod, int ba
     NttArgs a(vec, (unsigned)size, (unsigned)batches, (unsigned)root, (unsigned)mod, 2);
     a.inverse = true;
     autoNtt(a);
+}
+
+// nttCt: natural order in, the transform in bit-reversed order out, in place.
+inline void nttCt(std::vector<int> &vec, int root, int mod) {
+    float ms = 0.f;
+    check(stntt_auto_ntt_i32_ex(defaultContext().get(), vec.data(), vec.size(), (uint32_t)vec.size(), 1,
+                                (uint32_t)root, (uint32_t)mod, STNTT_FORWARD, STNTT_BITREV_OUT, &ms));
+}
+
+// inttGs: bit-reversed order in, the inverse (times n^-1) in natural order
+// out, in place; root is the forward root, as in the reference.
+inline void inttGs(std::vector<int> &vec, int root, int mod) {
+    float ms = 0.f;
+    check(stntt_auto_ntt_i32_ex(defaultContext().get(), vec.data(), vec.size(), (uint32_t)vec.size(), 1,
+                                (uint32_t)root, (uint32_t)mod, STNTT_INVERSE, STNTT_BITREV_IN, &ms));
+}
+
+// pwcNtt: vec0 <- cyclic convolution of vec0 and vec1, vec1 <- nttCt(vec1).
+inline void pwcNtt(std::vector<int> &vec0, std::vector<int> &vec1, int root, int mod) {
+    if (vec0.size() != vec1.size()) throw std::invalid_argument("vec0 and vec1 differ in size");
+    float ms = 0.f;
+    check(stntt_auto_polymul_i32(defaultContext().get(), vec0.data(), vec1.data(), (uint32_t)vec0.size(),
+                                 (uint32_t)root, (uint32_t)mod, &ms));
+}
+
+// elemWiseMul: s0[i] = s0[i] * s1[i] mod p for i < s0.size() <= s1.size(),
+// values in [0, mod).  Host-side, like the reference (it is one pass of
+// multiplies; the fused device form is inside pwcNtt).
+template <typename T>
+void elemWiseMul(std::span<T> s0, std::span<T> s1, size_t mod) {
+    if (s1.size() < s0.size()) throw std::out_of_range("s1 is shorter than s0");
+    for (size_t i = 0; i < s0.size(); ++i)
+        s0[i] = (T)(((unsigned __int128)(uint64_t)s0[i] * (uint64_t)s1[i]) % mod);
+}
+
+// pwc_convolution: the cyclic convolution of two positive vectors of equal
+// power-of-two length, with (root, mod) = findParams(len, max^2 len + 1) so
+// no output wraps.  The reference forms max^2 len + 1 in int; here it is
+// exact and a modulus of 2^31 or more is invalid_argument.
+inline std::tuple<std::vector<int>, int, int> pwc_convolution(std::span<const int> s0, std::span<const int> s1) {
+    if (s0.empty() || s0.size() != s1.size()) throw std::invalid_argument("s0 and s1 must be non-empty, equal length");
+    int64_t max_val = std::max(*std::max_element(s0.begin(), s0.end()), *std::max_element(s1.begin(), s1.end()));
+    uint64_t min_mod = (uint64_t)(max_val * max_val) * s0.size() + 1;
+    if (min_mod >= (1ull << 31)) throw std::invalid_argument("max^2 * len + 1 needs a modulus of 2^31 or more");
+    auto [root, mod] = findParams(s0.size(), min_mod);
+    std::vector<int> vec0(s0.begin(), s0.end()), vec1(s1.begin(), s1.end());
+    pwcNtt(vec0, vec1, (int)root, (int)mod);
+    return {vec0, (int)root, (int)mod};
 }
 
 // Goldilocks (P = 2^64 - 2^32 + 1), which the reference declares but does

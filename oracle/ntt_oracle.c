@@ -502,6 +502,80 @@ int orc_inverse_gl_batch(uint64_t *vec, unsigned log2n, uint64_t batches) {
     return 0;
 }
 
+static void bitrev_u64(uint64_t *v, uint64_t n) {
+    unsigned lv = ilog2(n);
+    for (size_t i = 0; i < n; ++i) {
+        size_t j = orc_bit_reverse(i, lv);
+        if (j > i) {
+            uint64_t t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+    }
+}
+
+/* Goldilocks twin of orc_ntt_ct_u32: the loop of src/ntt/ntt_cpu.cpp:11-42
+   (bit-reversed table of w^i, i < n/2; right = v[l] T[k]; k per block) */
+int orc_ntt_ct_gl(uint64_t *vec, unsigned log2n) {
+    if (log2n > 32) return -2;
+    uint64_t n = 1ull << log2n;
+    uint64_t *T = (uint64_t *)malloc((n / 2 + 1) * sizeof(uint64_t));
+    if (!T) return -2;
+    orc_twiddles_gl(T, log2n, 0);
+    if (n >= 2) bitrev_u64(T, n / 2);
+    for (uint64_t i = 0; i < n; ++i) vec[i] %= ORC_GL_P;
+    for (uint64_t size = n; size >= 2; size /= 2) {
+        uint64_t half = size / 2, k = 0;
+        for (uint64_t i = 0; i < n; i += size) {
+            for (uint64_t j = i; j < i + half; ++j) {
+                uint64_t l = j + half;
+                uint64_t left = vec[j], right = orc_gl_mul(vec[l], T[k]);
+                vec[j] = orc_gl_add(left, right);
+                vec[l] = orc_gl_sub(left, right);
+            }
+            k++;
+        }
+    }
+    free(T);
+    return 0;
+}
+
+/* Goldilocks twin of orc_intt_gs_u32: src/ntt/ntt_cpu.cpp:130-164 */
+int orc_intt_gs_gl(uint64_t *vec, unsigned log2n) {
+    if (log2n > 32) return -2;
+    uint64_t n = 1ull << log2n;
+    uint64_t *T = (uint64_t *)malloc((n / 2 + 1) * sizeof(uint64_t));
+    if (!T) return -2;
+    orc_twiddles_gl(T, log2n, 1);
+    if (n >= 2) bitrev_u64(T, n / 2);
+    for (uint64_t i = 0; i < n; ++i) vec[i] %= ORC_GL_P;
+    for (uint64_t size = 2; size <= n; size *= 2) {
+        uint64_t half = size / 2, k = 0;
+        for (uint64_t i = 0; i < n; i += size) {
+            for (uint64_t j = i; j < i + half; ++j) {
+                uint64_t l = j + half;
+                uint64_t left = vec[j], right = vec[l];
+                vec[j] = orc_gl_add(left, right);
+                vec[l] = orc_gl_mul(orc_gl_sub(left, right), T[k]);
+            }
+            k++;
+        }
+    }
+    uint64_t scaler = orc_invmod(n % ORC_GL_P, ORC_GL_P);
+    for (uint64_t i = 0; i < n; ++i) vec[i] = orc_gl_mul(vec[i], scaler);
+    free(T);
+    return 0;
+}
+
+/* Goldilocks twin of orc_pwc_ntt_u32 (src/ntt/ntt_cpu.cpp:167-174) */
+int orc_pwc_ntt_gl(uint64_t *a, uint64_t *b, unsigned log2n) {
+    int rc = orc_ntt_ct_gl(a, log2n);
+    if (!rc) rc = orc_ntt_ct_gl(b, log2n);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < (1ull << log2n); ++i) a[i] = orc_gl_mul(a[i], b[i]);
+    return orc_intt_gs_gl(a, log2n);
+}
+
 int orc_naive_dft_gl(const uint64_t *in, uint64_t *out, unsigned log2n, int inverse) {
     if (log2n > 32) return -2;
     uint64_t n = 1ull << log2n, root = orc_gl_root(log2n);

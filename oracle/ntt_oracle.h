@@ -76,6 +76,11 @@ int orc_stockham_gl_batch(uint64_t *vec, unsigned log2n, uint64_t batches, int n
 int orc_inverse_gl(uint64_t *vec, unsigned log2n);
 int orc_inverse_gl_batch(uint64_t *vec, unsigned log2n, uint64_t batches);
 int orc_naive_dft_gl(const uint64_t *in, uint64_t *out, unsigned log2n, int inverse);
+/* Goldilocks twins of the CT / GS / convolution loops above (same
+   ntt_cpu.cpp lines, Goldilocks arithmetic) */
+int orc_ntt_ct_gl(uint64_t *vec, unsigned log2n);
+int orc_intt_gs_gl(uint64_t *vec, unsigned log2n);
+int orc_pwc_ntt_gl(uint64_t *a, uint64_t *b, unsigned log2n);
 
 /* ---- sampled outputs of the textbook DFT, O(n) per output ------------- */
 int orc_dft_points_gl(const uint64_t *x, unsigned log2n, int inverse, const uint64_t *ks, uint64_t count,

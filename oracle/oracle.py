@@ -73,6 +73,9 @@ def lib() -> ctypes.CDLL:
         L.orc_inverse_gl.argtypes = [_u64p, _c.c_uint]
         L.orc_inverse_gl_batch.argtypes = [_u64p, _c.c_uint, _U64]
         L.orc_naive_dft_gl.argtypes = [_u64p, _u64p, _c.c_uint, _c.c_int]
+        L.orc_ntt_ct_gl.argtypes = [_u64p, _c.c_uint]
+        L.orc_intt_gs_gl.argtypes = [_u64p, _c.c_uint]
+        L.orc_pwc_ntt_gl.argtypes = [_u64p, _u64p, _c.c_uint]
         L.orc_dft_points_gl.argtypes = [_u64p, _c.c_uint, _c.c_int, _u64p, _U64, _u64p]
         L.orc_dft_points_u32.argtypes = [_u32p, _U64, _U32, _U32, _c.c_int, _u64p, _U64, _u32p]
         _lib = L
@@ -190,6 +193,25 @@ def stockham_gl(x: np.ndarray, log2n: int, batches: int = 1, nsteps: int = -1) -
     v = np.ascontiguousarray(x, dtype=np.uint64).copy()
     _check(lib().orc_stockham_gl_batch(v, log2n, batches, nsteps), "stockham_gl")
     return v
+
+
+def ntt_ct_gl(x: np.ndarray, log2n: int) -> np.ndarray:
+    v = np.ascontiguousarray(x, dtype=np.uint64).copy()
+    _check(lib().orc_ntt_ct_gl(v, log2n), "ntt_ct_gl")
+    return v
+
+
+def intt_gs_gl(x: np.ndarray, log2n: int) -> np.ndarray:
+    v = np.ascontiguousarray(x, dtype=np.uint64).copy()
+    _check(lib().orc_intt_gs_gl(v, log2n), "intt_gs_gl")
+    return v
+
+
+def pwc_ntt_gl(a: np.ndarray, b: np.ndarray, log2n: int) -> np.ndarray:
+    va = np.ascontiguousarray(a, dtype=np.uint64).copy()
+    vb = np.ascontiguousarray(b, dtype=np.uint64).copy()
+    _check(lib().orc_pwc_ntt_gl(va, vb, log2n), "pwc_ntt_gl")
+    return va
 
 
 def inverse_gl(x: np.ndarray, log2n: int, batches: int = 1) -> np.ndarray:

@@ -4,4 +4,5 @@ The compute path is libstntt.so (hand-written HIP kernels behind a C ABI,
 include/stntt.h); this package is its Python host side.
 """
 from .ntt import (GL_P, Batch, Context, DeviceBuffer, NttArgs, auto_ntt, default_context, find_params,  # noqa: F401
-                  Server, ntt_goldilocks, ntt_stockham, set_batch_split, set_pass_split, version)
+                  Server, elem_wise_mul, intt_gs, ntt_ct, ntt_goldilocks, ntt_stockham, pwc_convolution, pwc_ntt,
+                  set_batch_split, set_pass_split, version)

@@ -25,6 +25,8 @@ FIELD_GOLDILOCKS = 1
 FORWARD = 0
 INVERSE = 1
 SIGNED_INPUT = 1
+BITREV_IN = 2
+BITREV_OUT = 4
 
 _c = ctypes
 _vp = _c.c_void_p
@@ -45,8 +47,15 @@ SIGNATURES = [
     ("stntt_twiddles_goldilocks", _int, [_vp, _u32, _int, _vp]),
     ("stntt_ntt_u32", _int, [_vp, _vp, _u32, _u64, _u32, _u32, _int, _u32, _vp]),
     ("stntt_ntt_goldilocks", _int, [_vp, _vp, _u32, _u64, _int, _vp]),
+    ("stntt_ntt_goldilocks_ex", _int, [_vp, _vp, _u32, _u64, _int, _u32, _vp]),
+    ("stntt_pointwise_u32", _int, [_vp, _vp, _vp, _u64, _u32, _vp]),
+    ("stntt_pointwise_goldilocks", _int, [_vp, _vp, _vp, _u64, _vp]),
+    ("stntt_polymul_u32", _int, [_vp, _vp, _vp, _u32, _u64, _u32, _u32, _u32, _vp]),
+    ("stntt_polymul_goldilocks", _int, [_vp, _vp, _vp, _u32, _u64, _vp]),
     ("stntt_auto_ntt_i32", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
     ("stntt_auto_ntt_goldilocks", _int, [_vp, _vp, _sz, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
+    ("stntt_auto_ntt_i32_ex", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _u32, _int, _u32, _c.POINTER(_c.c_float)]),
+    ("stntt_auto_polymul_i32", _int, [_vp, _vp, _vp, _u32, _u32, _u32, _c.POINTER(_c.c_float)]),
     ("stntt_batch_create", _int, [_vp, _vp, _sz, _c.POINTER(_vp)]),
     ("stntt_batch_run", _int, [_vp, _vp]),
     ("stntt_batch_capture", _int, [_vp, _vp]),

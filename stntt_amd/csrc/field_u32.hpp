@@ -44,6 +44,26 @@ U32_HD uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
     return r < t ? r : t;
 }
 
+// high 64 bits of a 64 x 64 product
+U32_HD uint64_t u64_mulhi(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// a * b mod p for a, b < p (neither a twiddle with a companion): Barrett
+// with mu = floor(2^64 / p).  x = ab < 2^62; q = floor(x mu / 2^64) is
+// floor(x / p) or one less, so x - q p lies in [0, 2p) and fits 32 bits.
+U32_HD uint32_t u32_mul_any(uint32_t a, uint32_t b, uint32_t p, uint64_t mu) {
+    uint64_t x = (uint64_t)a * b;
+    uint64_t q = u64_mulhi(x, mu);
+    uint32_t r = (uint32_t)x - (uint32_t)q * p;
+    uint32_t t = r - p;
+    return r < t ? r : t;
+}
+
 // Reduce an arbitrary 32-bit input to [0, p).  bar = floor(2^32 / p);
 // x - floor(x * bar / 2^32) * p lies in [0, 2p).  For signed inputs a word
 // with the top bit set stands for x - 2^32, i.e. minus (2^32 mod p) = c32;

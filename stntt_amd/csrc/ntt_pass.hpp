@@ -69,7 +69,12 @@ enum : uint32_t {
     PASS_LAST = 2u,    // canonicalise outputs, apply n^-1 for the inverse
     PASS_SIGNED = 4u,  // 32-bit inputs are signed ints (reference int storage)
     PASS_SCALE = 8u,   // multiply by n^-1 (inverse)
+    PASS_BITREV_IN = 16u,   // input in bit-reversed order (single-pass kernel)
+    PASS_BITREV_OUT = 32u,  // output in bit-reversed order (single-pass kernel)
 };
+
+// bit reversal of the low `bits` bits of x, 1 <= bits <= 32
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits) { return __brev(x) >> (32 - bits); }
 
 struct FieldGL {
     using T = uint64_t;
@@ -85,6 +90,8 @@ struct FieldGL {
     static __device__ __forceinline__ T add(T a, T b, const Rt &) { return gl_add(a, b); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &) { return gl_sub(a, b); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &) { return gl_mul(a, w); }
+    // product of two field elements (pointwise spectra)
+    static __device__ __forceinline__ T mulg(T a, T b, const Rt &) { return gl_mul(a, b); }
     template <int LR, int DIR>
     static __device__ __forceinline__ Roots load_roots(const Tw *, int) {
         return {};
@@ -114,6 +121,7 @@ struct FieldU32 {
         uint32_t bar;  // floor(2^32 / p)
         uint32_t c32;  // 2^32 mod p
         U32Tw ninv;    // n^-1 with Shoup companion
+        uint64_t mu;   // floor(2^64 / p), for products of two data words
     };
     struct Roots {
         U32Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
@@ -124,6 +132,7 @@ struct FieldU32 {
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32_sub(a, b, rt.p); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return u32_mul(a, w, rt.p); }
+    static __device__ __forceinline__ T mulg(T a, T b, const Rt &rt) { return u32_mul_any(a, b, rt.p, rt.mu); }
     // omega_r^j = hi[j << (log_rmax - LR)] (hi[e] = omega_Rmax^e)
     template <int LR, int DIR>
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
@@ -190,6 +199,9 @@ struct PassArgs {
     uint32_t tiles_per_transform;
     uint32_t flags;
     typename FP::Rt rt;
+    // first pass only: when set, input word i is multiplied by mul_in[i]
+    // before the transform (a pointwise product fused into the load)
+    const typename FP::T *mul_in;
 };
 
 // w^e for e < n through the two factor tables
@@ -301,7 +313,15 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
             c = idx >> LOGR;
         }
         T x = 0;
-        if ((uint32_t)c < ncols) x = FP::load_in(a.in[in_base + c * cs_in + m * ms_in], a.rt, a.flags);
+        if ((uint32_t)c < ncols) {
+            const uint64_t gi = in_base + c * cs_in + m * ms_in;
+            x = FP::load_in(a.in[gi], a.rt, a.flags);
+            if (a.mul_in) x = FP::mulg(x, a.mul_in[gi], a.rt);
+        }
+        // bit-reversed input: word m of a transform is DFT input bitrev(m)
+        if constexpr (SINGLE) {
+            if (a.flags & PASS_BITREV_IN) m = (int)brev_bits((uint32_t)m, LOGR);
+        }
         lds[m * C + c] = x;
     }
     __syncthreads();
@@ -321,7 +341,12 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
             c = idx >> LOGR;
         }
         if ((uint32_t)c >= ncols) continue;
-        T x = lds[k * C + c];
+        // bit-reversed output: word k of a transform is DFT output bitrev(k)
+        int kk = k;
+        if constexpr (SINGLE) {
+            if (a.flags & PASS_BITREV_OUT) kk = (int)brev_bits((uint32_t)k, LOGR);
+        }
+        T x = lds[kk * C + c];
         if constexpr (!SINGLE) {
             // s J = t - j: with J = 1 it is t itself, otherwise the tile's s J
             uint64_t sJ = j_is_one ? (t0 + c) : ((t0 >> a.log_j) << a.log_j);

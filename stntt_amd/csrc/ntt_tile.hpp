@@ -59,7 +59,18 @@ constexpr int tile_threads() {
 // R * C words, twl R twiddles.
 // J1: first pass of a schedule (J = 1): outputs of a column are contiguous
 // (y[R t + k]); otherwise J >= C and a row of C outputs is contiguous.
-template <class FP, int LOGR, int DIR, bool J1, int NT>
+//
+// PERM_IN (first pass only): the input is in bit-reversed order.  Word
+// q = t + m n/R of the transform sits at bitrev_L(q) = bitrev(t) R +
+// bitrev_R(m), so column t is the contiguous R-word chunk at bitrev(t) R with
+// its rows permuted; the tile is staged through LDS so global reads stay
+// whole cache lines.  A pointwise operand (mul_in) is applied there too.
+// PERM_OUT (last pass only, J R = n): the output goes to bit-reversed
+// positions: DFT output k of column t lands at bitrev(t) R + bitrev_R(k), a
+// contiguous chunk per column, written back through LDS.
+enum : int { PERM_NONE = 0, PERM_IN = 1, PERM_OUT = 2 };
+
+template <class FP, int LOGR, int DIR, bool J1, int NT, int PERM = PERM_NONE>
 __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block, typename FP::T *lds,
                                           typename FP::Tw *twl) {
     using T = typename FP::T;
@@ -72,6 +83,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     constexpr int NG1 = (R / r1) * C, NG2 = r1 * C;  // groups of the two sub-steps
     constexpr int G1 = (NG1 + NT - 1) / NT, G2 = (NG2 + NT - 1) / NT;
     constexpr int TWL = (R + NT - 1) / NT;  // pass twiddles built per thread
+    constexpr int SP = (R * C + NT - 1) / NT;  // staged words per thread (PERM_IN / PERM_OUT)
+    static_assert(PERM != PERM_IN || (J1 && G1 == 1), "bit-reversed input: first pass, one group per thread");
+    static_assert(PERM != PERM_OUT || !J1, "bit-reversed output: last pass only");
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
@@ -95,6 +109,35 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     // adds one 32-bit element offset.
     const uint64_t mstep = (uint64_t)(R / r1) * stride_in;
     const auto roots = FP::template load_roots<LR1, DIR>(a.hi, a.log_rmax);
+    const uint32_t lt = logn - LOGR;  // bits of a column index t
+    if constexpr (PERM == PERM_IN) {
+        T sv[SP], su[SP];
+        const bool pw = a.mul_in != nullptr;
+        const T *mul = pw ? a.mul_in + base : nullptr;
+#pragma unroll
+        for (int i = 0; i < SP; ++i) {
+            const uint32_t idx = tid + i * NT;
+            if ((R * C) % NT == 0 || idx < (uint32_t)(R * C)) {
+                const uint32_t p = idx & (R - 1), c = idx >> LOGR;
+                const uint32_t off = brev_bits(t0 + c, lt) * R + p;
+                sv[i] = at_off(in, off);
+                if (pw) su[i] = at_off(mul, off);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < SP; ++i) {
+            const uint32_t idx = tid + i * NT;
+            if ((R * C) % NT == 0 || idx < (uint32_t)(R * C)) {
+                const uint32_t p = idx & (R - 1), c = idx >> LOGR;
+                T x = FP::load_in(sv[i], a.rt, a.flags);
+                if (pw) x = FP::mulg(x, su[i], a.rt);
+                const uint32_t m = brev_bits(p, LOGR);
+                lds[m * C + (c ^ (m & (C - 1)))] = x;
+            }
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int gi = 0; gi < G1; ++gi) {
         const uint32_t q = tid + gi * NT;
@@ -105,8 +148,16 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         T v[r1];
         Tw w1[r1];
         if (act) {
+            if constexpr (PERM == PERM_IN) {
 #pragma unroll
-            for (int m = 0; m < r1; ++m) v[m] = at_off(in + (uint64_t)m * mstep, lane);
+                for (int m = 0; m < r1; ++m) {
+                    const uint32_t row = g + m * (R / r1);
+                    v[m] = lds[row * C + (c ^ (row & (C - 1)))];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < r1; ++m) v[m] = at_off(in + (uint64_t)m * mstep, lane);
+            }
 #pragma unroll
             for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
         }
@@ -131,9 +182,12 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 }
             }
         }
+        if constexpr (PERM == PERM_IN) __syncthreads();  // staged rows all read before reuse
         if (act) {
+            if constexpr (PERM != PERM_IN) {
 #pragma unroll
-            for (int m = 0; m < r1; ++m) v[m] = FP::load_in(v[m], a.rt, a.flags);
+                for (int m = 0; m < r1; ++m) v[m] = FP::load_in(v[m], a.rt, a.flags);
+            }
             dft_regs<FP, LR1, DIR>(v, roots, a.rt);
 #pragma unroll
             for (int k = 0; k < r1; ++k) {
@@ -148,6 +202,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 
     // ---- sub-step 2: LDS -> registers -> DFT_r2 -> pass twiddle -> global --
     const auto roots2 = FP::template load_roots<LR2, DIR>(a.hi, a.log_rmax);
+    // PERM_OUT: outputs are kept in registers until the LDS tile is free
+    [[maybe_unused]] T xs[PERM == PERM_OUT ? G2 : 1][PERM == PERM_OUT ? r2 : 1];
+    [[maybe_unused]] uint32_t xc[G2], xg2[G2];
 #pragma unroll
     for (int gi = 0; gi < G2; ++gi) {
         const uint32_t q = tid + gi * NT;
@@ -215,20 +272,48 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 if (sJ != 0) x = FP::mul(x, twl[k], a.rt);  // uniform branch
             }
             if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
-            T *orow = J1 ? out + (uint64_t)t0 * R + k2 * r1
-                         : out + (uint64_t)sJ * R + (t0 - sJ) + ((uint64_t)(k2 * r1) << a.log_j);
-            at_off(orow, lane_out) = x;
+            if constexpr (PERM == PERM_OUT) {
+                xs[gi][k2] = x;
+            } else {
+                T *orow = J1 ? out + (uint64_t)t0 * R + k2 * r1
+                             : out + (uint64_t)sJ * R + (t0 - sJ) + ((uint64_t)(k2 * r1) << a.log_j);
+                at_off(orow, lane_out) = x;
+            }
+        }
+        xc[gi] = c;
+        xg2[gi] = g2;
+    }
+    if constexpr (PERM == PERM_OUT) {
+        __syncthreads();  // every sub-step 2 read of the tile is done
+#pragma unroll
+        for (int gi = 0; gi < G2; ++gi) {
+            if (NG2 % NT == 0 || tid + gi * NT < (uint32_t)NG2) {
+#pragma unroll
+                for (int k2 = 0; k2 < r2; ++k2) {
+                    const uint32_t p = brev_bits(k2 * r1 + xg2[gi], LOGR);
+                    lds[p * C + (xc[gi] ^ (p & (C - 1)))] = xs[gi][k2];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SP; ++i) {
+            const uint32_t idx = tid + i * NT;
+            if ((R * C) % NT == 0 || idx < (uint32_t)(R * C)) {
+                const uint32_t p = idx & (R - 1), c = idx >> LOGR;
+                at_off(out, brev_bits(t0 + c, lt) * R + p) = lds[p * C + (c ^ (p & (C - 1)))];
+            }
         }
     }
 }
 
-template <class FP, int LOGR, int DIR, bool J1>
+template <class FP, int LOGR, int DIR, bool J1, int PERM = PERM_NONE>
 __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(PassArgs<FP> a) {
     constexpr int C = 1 << tile_logc<FP>();
     constexpr int R = 1 << LOGR;
     __shared__ typename FP::T lds[R * C];
     __shared__ typename FP::Tw twl[J1 ? 1 : R];
-    tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>()>(a, blockIdx.x, lds, twl);
+    tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>(), PERM>(a, blockIdx.x, lds, twl);
 }
 
 }  // namespace stntt

@@ -203,6 +203,7 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
         plan->rt_u32.bar = (uint32_t)((1ull << 32) / p);
         plan->rt_u32.c32 = (uint32_t)((1ull << 32) % p);
         plan->rt_u32.ninv = shoup((uint32_t)invmod(n % p, p), p);
+        plan->rt_u32.mu = ~0ull / p;
         plan->bytes = tab.size() * sizeof(U32Tw);
         HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
         HIP_TRY(hipMemcpy(plan->d_tables, tab.data(), plan->bytes, hipMemcpyHostToDevice));
@@ -223,10 +224,16 @@ static void launch_single(const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
 }
 
 template <class FP, int DIR, int LOGR>
-static void launch_tile(bool first, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+static void launch_tile(bool first, bool perm, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
     constexpr int NT = tile_threads<FP, LOGR>();
-    if (first) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true>), dim3(grid), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false>), dim3(grid), dim3(NT), 0, s, a);
+    // perm: bit-reversed input on the first pass, bit-reversed output on the last
+    if (first) {
+        if (perm) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true, PERM_IN>), dim3(grid), dim3(NT), 0, s, a);
+        else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true>), dim3(grid), dim3(NT), 0, s, a);
+    } else {
+        if (perm) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false, PERM_OUT>), dim3(grid), dim3(NT), 0, s, a);
+        else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false>), dim3(grid), dim3(NT), 0, s, a);
+    }
 }
 
 template <class FP, int DIR, int... L>
@@ -236,22 +243,22 @@ static void launch_single_impl(int logr, const PassArgs<FP> &a, uint32_t grid, h
 }
 
 template <class FP, int DIR, int... L>
-static void launch_tile_impl(int logr, bool first, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
+static void launch_tile_impl(int logr, bool first, bool perm, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
                              std::integer_sequence<int, L...>) {
-    ((logr == L + MIN_LOGR_TILE ? (launch_tile<FP, DIR, L + MIN_LOGR_TILE>(first, a, grid, s), 0) : 0), ...);
+    ((logr == L + MIN_LOGR_TILE ? (launch_tile<FP, DIR, L + MIN_LOGR_TILE>(first, perm, a, grid, s), 0) : 0), ...);
 }
 
 template <class FP>
-static void launch_pass(int dir, bool single, bool first, int logr, const PassArgs<FP> &a, uint32_t grid,
+static void launch_pass(int dir, bool single, bool first, bool perm, int logr, const PassArgs<FP> &a, uint32_t grid,
                         hipStream_t s) {
     using SeqS = std::make_integer_sequence<int, MAX_LOGR_SINGLE<FP>>;
     using SeqT = std::make_integer_sequence<int, MAX_LOGR_TILE - MIN_LOGR_TILE + 1>;
     if (dir == 0) {
         if (single) launch_single_impl<FP, 0>(logr, a, grid, s, SeqS{});
-        else launch_tile_impl<FP, 0>(logr, first, a, grid, s, SeqT{});
+        else launch_tile_impl<FP, 0>(logr, first, perm, a, grid, s, SeqT{});
     } else {
         if (single) launch_single_impl<FP, 1>(logr, a, grid, s, SeqS{});
-        else launch_tile_impl<FP, 1>(logr, first, a, grid, s, SeqT{});
+        else launch_tile_impl<FP, 1>(logr, first, perm, a, grid, s, SeqT{});
     }
 }
 
@@ -262,9 +269,12 @@ __global__ void copy_kernel(const typename FP::T *in, typename FP::T *out, uint6
 }
 
 // Runs the pass schedule of `plan` over d_data[batches][n] in place.
+// flags: STNTT_SIGNED_INPUT, STNTT_BITREV_IN, STNTT_BITREV_OUT.  mul_in, when
+// set, is multiplied into the input word by word as the first pass loads it
+// (with bit-reversed input only on multi-pass schedules).
 template <class FP>
 static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_data, uint64_t batches,
-                         uint32_t flags, hipStream_t s) {
+                         uint32_t flags, hipStream_t s, const typename FP::T *mul_in = nullptr) {
     using T = typename FP::T;
     const uint32_t L = plan.key.log_n;
     if (batches == 0) return STNTT_OK;
@@ -278,6 +288,9 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         }
     }
     const bool inv = plan.key.dir == STNTT_INVERSE;
+    const bool rev_in = flags & STNTT_BITREV_IN, rev_out = flags & STNTT_BITREV_OUT;
+    if (mul_in && !sch.single && !rev_in)
+        return fail(STNTT_EINVAL, "a fused pointwise operand needs bit-reversed input on multi-pass sizes");
     // intermediates alternate between two scratch halves so that the last
     // pass always lands in the caller's buffer (no copy-back)
     T *scratch[2] = {nullptr, nullptr};
@@ -314,6 +327,13 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         a.log_j = log_j;
         a.flags = (first ? (PASS_FIRST | (flags & STNTT_SIGNED_INPUT ? PASS_SIGNED : 0)) : 0) |
                   (last ? (PASS_LAST | (inv ? PASS_SCALE : 0)) : 0);
+        a.mul_in = first ? mul_in : nullptr;
+        bool perm = false;
+        if (sch.single) {
+            a.flags |= (rev_in ? PASS_BITREV_IN : 0) | (rev_out ? PASS_BITREV_OUT : 0);
+        } else {
+            perm = (first && rev_in) || (last && rev_out);
+        }
         uint32_t grid;
         if (sch.single) {
             uint64_t c = 1ull << pass_logc<FP>(logr, true);
@@ -322,7 +342,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> tile_logc<FP>());
             grid = (uint32_t)(batches * a.tiles_per_transform);
         }
-        launch_pass<FP>(plan.key.dir, sch.single, first, logr, a, grid, s);
+        launch_pass<FP>(plan.key.dir, sch.single, first, perm, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
         src = dst;
@@ -440,6 +460,8 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
             return fail(STNTT_EINVAL, "descriptor %zu: bad field %u", i, d.field);
         if (d.direction > 1) return fail(STNTT_EINVAL, "descriptor %zu: bad direction %u", i, d.direction);
         if (d.log2n > 18) return fail(STNTT_EINVAL, "descriptor %zu: log2n %u above the batched limit 18", i, d.log2n);
+        if (d.flags & ~STNTT_SIGNED_INPUT)
+            return fail(STNTT_EINVAL, "descriptor %zu: flags 0x%x (batches take natural order only)", i, d.flags);
         if (!d.src || !d.dst) return fail(STNTT_EINVAL, "descriptor %zu: null buffer", i);
         const Plan *plan;
         int rc;
@@ -519,6 +541,7 @@ static int server_check_request(const stntt_request &r) {
     if (r.field != STNTT_FIELD_U32 && r.field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");
     if (r.direction > 1) return fail(STNTT_EINVAL, "bad direction");
     if (!r.host_in || !r.host_out) return fail(STNTT_EINVAL, "null host buffer");
+    if (r.flags & ~STNTT_SIGNED_INPUT) return fail(STNTT_EINVAL, "flags 0x%x (served in natural order only)", r.flags);
     if (r.field == STNTT_FIELD_GOLDILOCKS) {
         if (r.log2n > 28) return fail(STNTT_EINVAL, "log2n %u too large for serving", r.log2n);
         return STNTT_OK;
@@ -700,6 +723,36 @@ using namespace stntt;
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// pointwise products and polynomial multiplication
+// ---------------------------------------------------------------------------
+template <class FP>
+__global__ void pointwise_kernel(typename FP::T *a, const typename FP::T *b, uint64_t count, typename FP::Rt rt) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) a[i] = FP::mulg(a[i], b[i], rt);
+}
+
+template <class FP>
+static int launch_pointwise(typename FP::T *a, const typename FP::T *b, uint64_t count, typename FP::Rt rt,
+                            hipStream_t s) {
+    const uint64_t blocks = (count + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) return fail(STNTT_EINVAL, "count %llu too large", (unsigned long long)count);
+    if (blocks) hipLaunchKernelGGL(pointwise_kernel<FP>, dim3((uint32_t)blocks), dim3(256), 0, s, a, b, count, rt);
+    HIP_TRY(hipGetLastError());
+    return STNTT_OK;
+}
+
+// forward(a), forward(b) to bit-reversed order, then the inverse from
+// bit-reversed order with b multiplied in as the first pass loads a
+template <class FP>
+static int polymul_run(stntt_ctx *ctx, const Plan &fwd, const Plan &inv, typename FP::T *a, typename FP::T *b,
+                       uint64_t batches, uint32_t flags, hipStream_t s) {
+    int rc = run_transform<FP>(ctx, fwd, a, batches, flags | STNTT_BITREV_OUT, s);
+    if (!rc) rc = run_transform<FP>(ctx, fwd, b, batches, flags | STNTT_BITREV_OUT, s);
+    if (!rc) rc = run_transform<FP>(ctx, inv, a, batches, STNTT_BITREV_IN, s, b);
+    return rc;
+}
+
 extern "C" {
 
 const char *stntt_version(void) { return "stntt-mi355x 0.1 (gfx950)"; }
@@ -784,9 +837,12 @@ int stntt_find_params(uint64_t n, uint64_t min_mod, uint64_t *root, uint64_t *mo
     return STNTT_OK;
 }
 
+static constexpr uint32_t TRANSFORM_FLAGS = STNTT_SIGNED_INPUT | STNTT_BITREV_IN | STNTT_BITREV_OUT;
+
 int stntt_ntt_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t root, uint32_t mod,
                   int direction, uint32_t flags, stntt_stream stream) {
     if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~TRANSFORM_FLAGS) return fail(STNTT_EINVAL, "unknown flags 0x%x", flags);
     int rc = check_u32_params(log2n, root, mod, direction);
     if (rc) return rc;
     if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
@@ -799,7 +855,14 @@ int stntt_ntt_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t bat
 
 int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
                          stntt_stream stream) {
+    return stntt_ntt_goldilocks_ex(ctx, d_data, log2n, batches, direction, 0, stream);
+}
+
+int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                            uint32_t flags, stntt_stream stream) {
     if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~(STNTT_BITREV_IN | STNTT_BITREV_OUT))
+        return fail(STNTT_EINVAL, "flags 0x%x (Goldilocks takes the order flags only)", flags);
     if (log2n > 32) return fail(STNTT_EINVAL, "log2n %u exceeds the 2-adicity 32 of P - 1", log2n);
     if (direction != STNTT_FORWARD && direction != STNTT_INVERSE)
         return fail(STNTT_EINVAL, "bad direction %d", direction);
@@ -809,7 +872,59 @@ int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint6
     const Plan *plan;
     if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, direction, &plan))) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    return run_transform<FieldGL>(ctx, *plan, d_data, batches, 0, s);
+    return run_transform<FieldGL>(ctx, *plan, d_data, batches, flags, s);
+}
+
+int stntt_pointwise_u32(stntt_ctx *ctx, uint32_t *d_a, const uint32_t *d_b, uint64_t count, uint32_t mod,
+                        stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (mod < 2 || mod >= (1u << 31)) return fail(STNTT_EINVAL, "mod %u outside [2, 2^31)", mod);
+    if (count && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    FieldU32::Rt rt{};
+    rt.p = mod;
+    rt.mu = ~0ull / mod;
+    return launch_pointwise<FieldU32>(d_a, d_b, count, rt, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_pointwise_goldilocks(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_b, uint64_t count,
+                               stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (count && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    return launch_pointwise<FieldGL>(d_a, d_b, count, FieldGL::Rt{}, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_polymul_u32(stntt_ctx *ctx, uint32_t *d_a, uint32_t *d_b, uint32_t log2n, uint64_t batches,
+                      uint32_t root, uint32_t mod, uint32_t flags, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~STNTT_SIGNED_INPUT) return fail(STNTT_EINVAL, "flags 0x%x (only STNTT_SIGNED_INPUT)", flags);
+    int rc = check_u32_params(log2n, root, mod, STNTT_INVERSE);
+    if (rc) return rc;
+    if (batches && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    if (d_a == d_b && batches) return fail(STNTT_EINVAL, "a and b must be distinct buffers");
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *fwd, *inv;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U32, log2n, root % mod, mod, STNTT_FORWARD, &fwd))) return rc;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U32, log2n, root % mod, mod, STNTT_INVERSE, &inv))) return rc;
+    return polymul_run<FieldU32>(ctx, *fwd, *inv, d_a, d_b, batches, flags,
+                                 stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_polymul_goldilocks(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint32_t log2n, uint64_t batches,
+                             stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (log2n > 32) return fail(STNTT_EINVAL, "log2n %u exceeds the 2-adicity 32 of P - 1", log2n);
+    if (batches && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    if (d_a == d_b && batches) return fail(STNTT_EINVAL, "a and b must be distinct buffers");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *fwd, *inv;
+    if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, STNTT_FORWARD, &fwd))) return rc;
+    if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, STNTT_INVERSE, &inv))) return rc;
+    return polymul_run<FieldGL>(ctx, *fwd, *inv, d_a, d_b, batches, 0, stream ? (hipStream_t)stream : ctx->stream);
 }
 
 int stntt_twiddles_u32(stntt_ctx *ctx, uint64_t n, uint32_t root, uint32_t mod, int direction, uint32_t *host_out) {
@@ -857,6 +972,40 @@ int stntt_auto_ntt_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint3
     return auto_ntt_common<int32_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, int32_t *d) {
         return stntt_ntt_u32(ctx, (uint32_t *)d, L, batches, root, mod, direction, STNTT_SIGNED_INPUT, nullptr);
     });
+}
+
+int stntt_auto_ntt_i32_ex(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                          uint32_t root, uint32_t mod, int direction, uint32_t flags, float *kernel_ms) {
+    return auto_ntt_common<int32_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, int32_t *d) {
+        return stntt_ntt_u32(ctx, (uint32_t *)d, L, batches, root, mod, direction, flags | STNTT_SIGNED_INPUT,
+                             nullptr);
+    });
+}
+
+int stntt_auto_polymul_i32(stntt_ctx *ctx, int32_t *host_a, int32_t *host_b, uint32_t n, uint32_t root, uint32_t mod,
+                           float *kernel_ms) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (!host_a || !host_b) return fail(STNTT_EINVAL, "null host vector");
+    uint32_t L;
+    int rc = log2_exact(n, &L);
+    if (rc) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    const size_t bytes = (size_t)n * 4;
+    if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, 2 * bytes))) return rc;
+    int32_t *da = (int32_t *)ctx->staging, *db = da + n;
+    HIP_TRY(hipMemcpyAsync(da, host_a, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(db, host_b, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+    if ((rc = stntt_polymul_u32(ctx, (uint32_t *)da, (uint32_t *)db, L, 1, root, mod, STNTT_SIGNED_INPUT, nullptr)))
+        return rc;
+    HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host_a, da, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host_b, db, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = ms;
+    return STNTT_OK;
 }
 
 int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,

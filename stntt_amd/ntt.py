@@ -107,17 +107,41 @@ class Context:
 
     # ---- transforms on device buffers (asynchronous) ----------------------
     def ntt_u32(self, buf: "DeviceBuffer | int", log2n: int, batches: int, root: int, mod: int,
-                inverse: bool = False, signed: bool = False, stream: int | None = None) -> None:
+                inverse: bool = False, signed: bool = False, stream: int | None = None,
+                bitrev_in: bool = False, bitrev_out: bool = False) -> None:
+        """In place.  bitrev_out: output in bit-reversed order (nttCt's);
+        bitrev_in: input in bit-reversed order (inttGs's)."""
         ptr = buf.ptr if isinstance(buf, DeviceBuffer) else buf
+        flags = (_lib.SIGNED_INPUT if signed else 0) | _order_flags(bitrev_in, bitrev_out)
         check(lib().stntt_ntt_u32(self.handle, ptr, log2n, batches, root, mod,
-                                  _lib.INVERSE if inverse else _lib.FORWARD,
-                                  _lib.SIGNED_INPUT if signed else 0, stream))
+                                  _lib.INVERSE if inverse else _lib.FORWARD, flags, stream))
 
     def ntt_goldilocks(self, buf: "DeviceBuffer | int", log2n: int, batches: int, inverse: bool = False,
-                       stream: int | None = None) -> None:
+                       stream: int | None = None, bitrev_in: bool = False, bitrev_out: bool = False) -> None:
         ptr = buf.ptr if isinstance(buf, DeviceBuffer) else buf
-        check(lib().stntt_ntt_goldilocks(self.handle, ptr, log2n, batches,
-                                         _lib.INVERSE if inverse else _lib.FORWARD, stream))
+        check(lib().stntt_ntt_goldilocks_ex(self.handle, ptr, log2n, batches,
+                                            _lib.INVERSE if inverse else _lib.FORWARD,
+                                            _order_flags(bitrev_in, bitrev_out), stream))
+
+    # ---- pointwise products and polynomial multiplication ----------------
+    def pointwise_u32(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int, mod: int,
+                      stream: int | None = None) -> None:
+        """a[i] = a[i] * b[i] mod p (elemWiseMul)."""
+        check(lib().stntt_pointwise_u32(self.handle, a.ptr, b.ptr, count, mod, stream))
+
+    def pointwise_goldilocks(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int,
+                             stream: int | None = None) -> None:
+        check(lib().stntt_pointwise_goldilocks(self.handle, a.ptr, b.ptr, count, stream))
+
+    def polymul_u32(self, a: "DeviceBuffer", b: "DeviceBuffer", log2n: int, batches: int, root: int, mod: int,
+                    signed: bool = False, stream: int | None = None) -> None:
+        """pwcNtt on device buffers: a <- a (*) b (cyclic), b <- nttCt(b)."""
+        check(lib().stntt_polymul_u32(self.handle, a.ptr, b.ptr, log2n, batches, root, mod,
+                                      _lib.SIGNED_INPUT if signed else 0, stream))
+
+    def polymul_goldilocks(self, a: "DeviceBuffer", b: "DeviceBuffer", log2n: int, batches: int,
+                           stream: int | None = None) -> None:
+        check(lib().stntt_polymul_goldilocks(self.handle, a.ptr, b.ptr, log2n, batches, stream))
 
     # ---- twiddle tables in the reference layout ---------------------------
     def twiddles_u32(self, n: int, root: int, mod: int, inverse: bool = False) -> np.ndarray:
@@ -129,6 +153,10 @@ class Context:
         out = np.zeros(max((1 << log2n) // 2, 1), dtype=np.uint64)
         check(lib().stntt_twiddles_goldilocks(self.handle, log2n, int(inverse), out.ctypes.data))
         return out[: (1 << log2n) // 2]
+
+
+def _order_flags(bitrev_in: bool, bitrev_out: bool) -> int:
+    return (_lib.BITREV_IN if bitrev_in else 0) | (_lib.BITREV_OUT if bitrev_out else 0)
 
 
 class DeviceBuffer:
@@ -246,6 +274,124 @@ def ntt_goldilocks(vec: np.ndarray, size: int, batches: int = 1, inverse: bool =
     """Goldilocks transform of vec[batches][size] (uint64) on the GPU, in place."""
     auto_ntt(NttArgs(vec, size, batches, 0, GL_P, field="goldilocks", inverse=inverse), ctx)
     return vec
+
+
+def _host_transform(vec: np.ndarray, inverse: bool, flags: int, root: int, mod: int, ctx: Context | None) -> float:
+    """One transform of the whole vector (length = power of two) in place."""
+    ctx = ctx or default_context()
+    if not isinstance(vec, np.ndarray) or not vec.flags.c_contiguous:
+        raise ValueError("vec must be a C-contiguous numpy array")
+    n = vec.size
+    L = _log2_exact(n)
+    direction = _lib.INVERSE if inverse else _lib.FORWARD
+    ms = ctypes.c_float()
+    if vec.dtype == np.int32:
+        check(lib().stntt_auto_ntt_i32_ex(ctx.handle, vec.ctypes.data, n, n, 1, root, mod, direction, flags,
+                                          ctypes.byref(ms)))
+        return ms.value
+    if vec.dtype not in (np.uint32, np.uint64):
+        raise ValueError("vectors are int32 / uint32 (32-bit field) or uint64 (Goldilocks)")
+    buf = ctx.upload(vec)
+    ctx.timer_start()
+    if vec.dtype == np.uint64:
+        check(lib().stntt_ntt_goldilocks_ex(ctx.handle, buf.ptr, L, 1, direction, flags, None))
+    else:
+        check(lib().stntt_ntt_u32(ctx.handle, buf.ptr, L, 1, root, mod, direction, flags, None))
+    ms.value = ctx.timer_stop()
+    buf.copy_to(vec)
+    buf.free()
+    return ms.value
+
+
+def ntt_ct(vec: np.ndarray, root: int = 0, mod: int = GL_P, ctx: Context | None = None) -> np.ndarray:
+    """nttCt(vec, root, mod) (src/ntt/ntt_cpu.cpp:11-42) on the GPU, in place:
+    natural order in, the transform in bit-reversed order out.  uint64
+    vectors are Goldilocks (root and mod ignored)."""
+    _host_transform(vec, False, _lib.BITREV_OUT, root, mod, ctx)
+    return vec
+
+
+def intt_gs(vec: np.ndarray, root: int = 0, mod: int = GL_P, ctx: Context | None = None) -> np.ndarray:
+    """inttGs(vec, root, mod) (src/ntt/ntt_cpu.cpp:130-164) on the GPU, in
+    place: bit-reversed order in, the inverse (scaled by n^-1) in natural
+    order out.  root is the forward root, as in the reference."""
+    _host_transform(vec, True, _lib.BITREV_IN, root, mod, ctx)
+    return vec
+
+
+def elem_wise_mul(s0: np.ndarray, s1: np.ndarray, mod: int = GL_P, ctx: Context | None = None) -> np.ndarray:
+    """elemWiseMul(s0, s1, mod) (src/ntt/ntt_util.h:28-33): s0[i] = s0[i] s1[i]
+    mod p for i < len(s0), in place; len(s0) <= len(s1)."""
+    ctx = ctx or default_context()
+    n = s0.size
+    if s1.size < n:
+        raise IndexError("s1 is shorter than s0")
+    if s0.dtype != s1.dtype or s0.dtype not in (np.uint32, np.uint64):
+        raise ValueError("s0 and s1 are both uint32 (32-bit field) or both uint64 (Goldilocks)")
+    if n == 0:
+        return s0
+    a, b = ctx.upload(s0), ctx.upload(np.ascontiguousarray(s1[:n]))
+    if s0.dtype == np.uint64:
+        ctx.pointwise_goldilocks(a, b, n)
+    else:
+        ctx.pointwise_u32(a, b, n, mod)
+    a.copy_to(s0)
+    a.free()
+    b.free()
+    return s0
+
+
+def pwc_ntt(vec0: np.ndarray, vec1: np.ndarray, root: int = 0, mod: int = GL_P,
+            ctx: Context | None = None) -> np.ndarray:
+    """pwcNtt(vec0, vec1, root, mod) (src/ntt/ntt_cpu.cpp:167-174) on the GPU:
+    vec0 <- cyclic convolution of vec0 and vec1, vec1 <- nttCt(vec1); both in
+    place.  int32 vectors follow the reference's int storage."""
+    ctx = ctx or default_context()
+    for v in (vec0, vec1):
+        if not isinstance(v, np.ndarray) or not v.flags.c_contiguous:
+            raise ValueError("vectors must be C-contiguous numpy arrays")
+    if vec0.size != vec1.size or vec0.dtype != vec1.dtype:
+        raise ValueError("vec0 and vec1 must have the same length and dtype")
+    n = vec0.size
+    L = _log2_exact(n)
+    if vec0.dtype == np.int32:
+        ms = ctypes.c_float()
+        check(lib().stntt_auto_polymul_i32(ctx.handle, vec0.ctypes.data, vec1.ctypes.data, n, root, mod,
+                                           ctypes.byref(ms)))
+        return vec0
+    if vec0.dtype not in (np.uint32, np.uint64):
+        raise ValueError("vectors are int32 / uint32 (32-bit field) or uint64 (Goldilocks)")
+    a, b = ctx.upload(vec0), ctx.upload(vec1)
+    if vec0.dtype == np.uint64:
+        ctx.polymul_goldilocks(a, b, L, 1)
+    else:
+        ctx.polymul_u32(a, b, L, 1, root, mod)
+    a.copy_to(vec0)
+    b.copy_to(vec1)
+    a.free()
+    b.free()
+    return vec0
+
+
+def pwc_convolution(s0: np.ndarray, s1: np.ndarray, ctx: Context | None = None) -> tuple[np.ndarray, int, int]:
+    """pwc_convolution(s0, s1) (src/ntt/ntt_cpu.cpp:180-192): the cyclic
+    convolution of two positive int vectors of equal power-of-two length,
+    with (root, mod) from findParams(len, max^2 len + 1) so that no output
+    wraps the modulus.  The inputs are not modified.  Returns (vec, root, mod).
+    The reference forms max^2 len + 1 in int; here it is exact and a value
+    of 2^31 or more is an error (the 32-bit field holds p < 2^31)."""
+    a = np.array(s0, dtype=np.int64)
+    b = np.array(s1, dtype=np.int64)
+    if a.size == 0 or a.size != b.size:
+        raise ValueError("s0 and s1 must be non-empty and of equal length")
+    max_val = int(max(a.max(), b.max()))
+    min_mod = max_val * max_val * a.size + 1
+    if min_mod >= 1 << 31:
+        raise ValueError(f"max^2 * len + 1 = {min_mod} needs a modulus of 2^31 or more")
+    root, mod = find_params(a.size, min_mod)
+    v0, v1 = a.astype(np.int32), b.astype(np.int32)
+    pwc_ntt(v0, v1, root, mod, ctx)
+    return v0, root, mod
 
 
 # --------------------------------------------------------------------------

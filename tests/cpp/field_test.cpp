@@ -73,7 +73,7 @@ int main() {
         CHECK(gl_sub_mul_pow2<96>(a, b) == ref_pow2(d, 96), "submul96");
         CHECK(gl_sub_mul_pow2<0>(a, b) == d, "submul0");
     }
-    // 32-bit field: Shoup product, add, sub, input reduction, for the
+    // 32-bit field: Shoup and Barrett products, add, sub, input reduction, for the
     // findParams primes and the largest prime below 2^31
     for (uint32_t p : {11u, 12289u, 65537u, 7340033u, 2013265921u, 2147483629u}) {
         uint32_t bar = (uint32_t)((1ull << 32) / p), c32 = (uint32_t)((1ull << 32) % p);
@@ -91,6 +91,8 @@ int main() {
                 CHECK(u32_sub(a, b, p) == (uint32_t)(((uint64_t)a + p - b) % p), "u32 sub");
                 U32Tw w{b, (uint32_t)(((uint64_t)b << 32) / p)};
                 CHECK(u32_mul(x, w, p) == (uint32_t)((uint64_t)x * b % p), "u32 mul %u * %u mod %u", x, b, p);
+                CHECK(u32_mul_any(a, b, p, ~0ull / p) == (uint32_t)((uint64_t)a * b % p), "u32 mul_any %u * %u mod %u",
+                      a, b, p);
             }
         }
     }

@@ -80,6 +80,35 @@ int main() {
     if (o != h) { std::printf("Goldilocks forward mismatch\n"); ++failures; }
     stntt::autoNttGoldilocks(h, 1 << 16, 1, true);
     if (g != h) { std::printf("Goldilocks round trip mismatch\n"); ++failures; }
+    // nttCt / inttGs / pwcNtt / pwc_convolution as the reference calls them
+    // (src/ntt/ntt_cpu.cpp:11-42, 130-192), against the oracle's restatements
+    for (size_t L : {3u, 10u, 15u}) {
+        size_t n = 1ull << L;
+        auto [root, mod] = stntt::findParams(n, 11);
+        std::vector<int> x(vec.begin(), vec.begin() + n), y(vec.begin() + n, vec.begin() + 2 * n);
+        std::vector<uint32_t> ox(x.begin(), x.end()), oy(y.begin(), y.end());
+        std::vector<int> ct = x;
+        stntt::nttCt(ct, (int)root, (int)mod);
+        orc_ntt_ct_u32(ox.data(), n, (uint32_t)root, (uint32_t)mod);
+        if (!std::equal(ct.begin(), ct.end(), ox.begin())) { std::printf("nttCt mismatch 2^%zu\n", L); ++failures; }
+        stntt::inttGs(ct, (int)root, (int)mod);
+        if (ct != x) { std::printf("inttGs round trip mismatch 2^%zu\n", L); ++failures; }
+        std::vector<uint32_t> pa(x.begin(), x.end()), pb(y.begin(), y.end());
+        orc_pwc_ntt_u32(pa.data(), pb.data(), n, (uint32_t)root, (uint32_t)mod);
+        std::vector<int> a = x, b = y;
+        stntt::pwcNtt(a, b, (int)root, (int)mod);
+        if (!std::equal(a.begin(), a.end(), pa.begin())) { std::printf("pwcNtt mismatch 2^%zu\n", L); ++failures; }
+        cases += 3;
+    }
+    {
+        std::vector<int> s0 = {1, 2, 3, 4}, s1 = {5, 6, 7, 8};
+        auto [res, root, mod] = stntt::pwc_convolution(s0, s1);
+        std::vector<int> want(4, 0);  // schoolbook cyclic product
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) want[(i + j) % 4] += s0[i] * s1[j];
+        if (res != want) { std::printf("pwc_convolution mismatch\n"); ++failures; }
+        ++cases;
+    }
     std::printf("%d cases, %d failures\n", cases, failures);
     return failures ? 1 : 0;
 }

@@ -255,6 +255,28 @@ def test_goldilocks_convolution_theorem():
     assert [int(v) for v in c] == want
 
 
+@pytest.mark.parametrize("L", range(1, 7))
+def test_goldilocks_ct_gs_match_bigint_dft(gl_naive, L):
+    """The Goldilocks twins of nttCt / inttGs against the Python big-int DFT:
+    CT gives the DFT in bit-reversed order, GS takes that order back to x."""
+    x, X = gl_naive[f"x_L{L}"], gl_naive[f"X_L{L}"]
+    perm = [O.lib().orc_bit_reverse(i, L) for i in range(1 << L)]
+    assert np.array_equal(O.ntt_ct_gl(x, L), X[perm])
+    assert np.array_equal(O.intt_gs_gl(X[perm], L), x % np.uint64(P))
+
+
+@pytest.mark.parametrize("L", [3, 6])
+def test_goldilocks_pwc_is_cyclic_convolution(L):
+    n = 1 << L
+    a = splitmix64(21 + L, n) % np.uint64(P)
+    b = splitmix64(31 + L, n) % np.uint64(P)
+    want = [0] * n
+    for i in range(n):
+        for j in range(n):
+            want[(i + j) % n] = (want[(i + j) % n] + int(a[i]) * int(b[j])) % P
+    assert [int(v) for v in O.pwc_ntt_gl(a, b, L)] == want
+
+
 def test_fixtures_discriminate_plausible_mistakes(ref_u32):
     """A wrong root (the inverse root), dropped autosort (bit-reversed output)
     or a transposed batch would each fail the fixture check above."""
