@@ -111,6 +111,93 @@ def bench_n20(S, ctx, steps: int) -> dict:
     return out
 
 
+def _time_steps(ctx, step, steps: int) -> float:
+    for _ in range(3):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_orders_polymul(S, ctx, steps: int) -> dict:
+    """Element orders at 2^24 (Goldilocks, device-resident) and polynomial
+    multiply (pwcNtt shape: two forward transforms to bit-reversed order,
+    pointwise product, inverse from bit-reversed order), the pointwise
+    product fused into the inverse's first pass versus a separate kernel.
+    Algorithmic bytes of one multiply: 2 n w per forward, 3 n w for the
+    inverse (reads a and b, writes a) = 7 n w."""
+    out = {}
+    L = 24
+    n = 1 << L
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, P, size=n, dtype=np.uint64)
+    buf = ctx.upload(x)
+    orders = {}
+    for name, kw in (("natural", {}), ("bitrev_out", {"bitrev_out": True}),
+                     ("bitrev_in", {"bitrev_in": True})):
+        orders[name] = round(_time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1, **kw), steps) * 1e3, 4)
+    out["goldilocks_2^24_forward_ms"] = orders
+    buf.free()
+
+    for field, L, batches in (("goldilocks", 24, 1), ("goldilocks", 20, 1), ("u32", 20, 1), ("u32", 16, 64)):
+        n = 1 << L
+        if field == "u32":
+            root, mod = S.find_params(n, 11)
+            a = rng.integers(0, mod, size=batches * n, dtype=np.uint64).astype(np.uint32)
+            b = rng.integers(0, mod, size=batches * n, dtype=np.uint64).astype(np.uint32)
+        else:
+            a = rng.integers(0, P, size=batches * n, dtype=np.uint64)
+            b = rng.integers(0, P, size=batches * n, dtype=np.uint64)
+        w = a.dtype.itemsize
+        da, db, ra, rb = ctx.upload(a), ctx.upload(b), ctx.alloc(a.nbytes), ctx.alloc(b.nbytes)
+
+        def reset():
+            ctx.copy(ra, da, a.nbytes)
+            ctx.copy(rb, db, b.nbytes)
+
+        def fused():
+            reset()
+            if field == "u32":
+                ctx.polymul_u32(ra, rb, L, batches, root, mod)
+            else:
+                ctx.polymul_goldilocks(ra, rb, L, batches)
+
+        def unfused():
+            reset()
+            if field == "u32":
+                ctx.ntt_u32(ra, L, batches, root, mod, bitrev_out=True)
+                ctx.ntt_u32(rb, L, batches, root, mod, bitrev_out=True)
+                ctx.pointwise_u32(ra, rb, batches * n, mod)
+                ctx.ntt_u32(ra, L, batches, root, mod, inverse=True, bitrev_in=True)
+            else:
+                ctx.ntt_goldilocks(ra, L, batches, bitrev_out=True)
+                ctx.ntt_goldilocks(rb, L, batches, bitrev_out=True)
+                ctx.pointwise_goldilocks(ra, rb, batches * n)
+                ctx.ntt_goldilocks(ra, L, batches, inverse=True, bitrev_in=True)
+
+        def copies():
+            reset()
+
+        t_copy = _time_steps(ctx, copies, steps)
+        t_fused = _time_steps(ctx, fused, steps) - t_copy
+        got_fused = ra.copy_to(np.empty_like(a))
+        t_unfused = _time_steps(ctx, unfused, steps) - t_copy
+        got_unfused = ra.copy_to(np.empty_like(a))
+        alg = 7 * batches * n * w
+        out[f"polymul_{field}_2^{L}_x{batches}"] = {
+            "fused_ms": round(t_fused * 1e3, 4), "unfused_ms": round(t_unfused * 1e3, 4),
+            "gelem_s": round(batches * n / t_fused / 1e9, 3),
+            "hbm_frac": round(alg / t_fused / 1e9 / HBM_PEAK_GBS, 4),
+            "fused_equals_unfused": bool(np.array_equal(got_fused, got_unfused)),
+            "note": "times exclude the two input-restoring device copies timed separately"}
+        for d in (da, db, ra, rb):
+            d.free()
+    return out
+
+
 def bench_batched(S, ctx, count: int, reps: int) -> dict:
     """4096 transforms, log2 n ~ U{10..18}, field and direction 50/50: one
     persistent launch (graph replay) versus per-transform plain launches."""
@@ -319,6 +406,8 @@ def main() -> int:
     if not args.no_extra:
         log("[bench] n=2^20 forward+inverse, both fields")
         out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
+        log("[bench] element orders and polynomial multiply")
+        out["orders_polymul"] = bench_orders_polymul(S, ctx, 20)
         log(f"[bench] batched mixed workload ({args.batch_count} transforms)")
         out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10)
         log("[bench] serving stream")
