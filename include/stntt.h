@@ -41,7 +41,11 @@ typedef enum stntt_status {
     STNTT_ENOTFOUND = 6 /* serving: unknown request id */
 } stntt_status;
 
-typedef enum stntt_field { STNTT_FIELD_U32 = 0, STNTT_FIELD_GOLDILOCKS = 1 } stntt_field;
+/* STNTT_FIELD_U32: the reference's 32-bit prime field (runtime root and mod);
+   STNTT_FIELD_GOLDILOCKS: P = 2^64 - 2^32 + 1;
+   STNTT_FIELD_U64: any odd prime below 2^64 with a runtime root (Montgomery
+   arithmetic); direct transforms, orders and polynomial multiply only. */
+typedef enum stntt_field { STNTT_FIELD_U32 = 0, STNTT_FIELD_GOLDILOCKS = 1, STNTT_FIELD_U64 = 2 } stntt_field;
 typedef enum stntt_direction { STNTT_FORWARD = 0, STNTT_INVERSE = 1 } stntt_direction;
 
 /* transform flags */
@@ -70,7 +74,10 @@ stntt_stream stntt_ctx_stream(stntt_ctx *ctx);
 
 /* ---- parameters (host only) ------------------------------------------- */
 /* Smallest prime p = k n + 1 >= min_mod and root = g^((p-1)/n), g the
-   smallest generator.  Replaces findParams (src/ntt/ntt_util.cpp:10-14). */
+   smallest generator.  Replaces findParams (src/ntt/ntt_util.cpp:10-14);
+   the same answers, over the whole 64-bit range (deterministic Miller-Rabin
+   and Pollard rho in place of the reference's trial division, whose pow
+   overflows above 2^32, ntt_util.h:72-96). */
 int stntt_find_params(uint64_t n, uint64_t min_mod, uint64_t *root, uint64_t *mod);
 
 /* ---- twiddle plans ------------------------------------------------------ */
@@ -95,6 +102,19 @@ int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint6
 int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
                             uint32_t flags, stntt_stream stream);
 
+/* Any odd prime p < 2^64 (STNTT_FIELD_U64): in place on d_data[batches][2^log2n]
+   with a root of order exactly n (root^(n/2) = -1 mod p), e.g. from
+   stntt_find_params(n, min_mod) with a 64-bit min_mod.  Every 64-bit input
+   word is reduced mod p exactly; outputs are in [0, p).  flags: the order
+   flags.  Twiddle plans for this field (and the others) are generated on the
+   device.  No counterpart in the reference (its parameter search cannot
+   produce 64-bit primes). */
+int stntt_ntt_u64(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, uint64_t root,
+                  uint64_t mod, int direction, uint32_t flags, stntt_stream stream);
+/* logical table T[i] = root^i mod p, i < n/2, as stntt_twiddles_u32 */
+int stntt_twiddles_u64(stntt_ctx *ctx, uint32_t log2n, uint64_t root, uint64_t mod, int direction,
+                       uint64_t *host_out);
+
 /* ---- pointwise products and polynomial multiplication ------------------ */
 /* a[i] = a[i] * b[i] mod p for i < count, a and b canonical (< p):
    elemWiseMul (src/ntt/ntt_util.h:28-33) on device buffers. */
@@ -113,6 +133,10 @@ int stntt_polymul_u32(stntt_ctx *ctx, uint32_t *d_a, uint32_t *d_b, uint32_t log
                       uint32_t root, uint32_t mod, uint32_t flags, stntt_stream stream);
 int stntt_polymul_goldilocks(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint32_t log2n, uint64_t batches,
                              stntt_stream stream);
+int stntt_pointwise_u64(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_b, uint64_t count, uint64_t mod,
+                        stntt_stream stream);
+int stntt_polymul_u64(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint32_t log2n, uint64_t batches,
+                      uint64_t root, uint64_t mod, stntt_stream stream);
 
 /* ---- transforms on host buffers (synchronous, autoNtt-shaped) ---------- */
 /* Copies host_vec[0 .. n*batches) to the device, transforms it, copies it

@@ -642,3 +642,157 @@ int orc_dft_points_u32(const uint32_t *x, uint64_t n, uint32_t root, uint32_t mo
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Any odd modulus below 2^64 (the 64-bit field): the same loops as the   */
+/* 32-bit functions above with 128-bit products.                          */
+/* ---------------------------------------------------------------------- */
+static inline uint64_t mm64_mul(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a * b) % m); }
+static inline uint64_t mm64_add(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a + b) % m); }
+static inline uint64_t mm64_sub(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)(((u128)a + m - b) % m); }
+
+int orc_twiddles_u64(uint64_t *out, uint64_t n, uint64_t root, uint64_t mod) {
+    uint64_t t = 1 % mod;
+    for (uint64_t i = 0; i < n / 2; ++i) {
+        out[i] = t;
+        t = mm64_mul(t, root, mod);
+    }
+    return 0;
+}
+
+/* src/ntt/ntt_cpu.cpp:46-85 over Z_mod */
+int orc_stockham_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, int nsteps) {
+    if (!is_pow2(n)) return -1;
+    int steps = (int)ilog2(n);
+    if (nsteps >= 0) steps = nsteps;
+    uint64_t *y = (uint64_t *)malloc(n * sizeof(uint64_t));
+    uint64_t *T = (uint64_t *)malloc((n / 2 + 1) * sizeof(uint64_t));
+    if (!y || !T) {
+        free(y);
+        free(T);
+        return -2;
+    }
+    orc_twiddles_u64(T, n, root % mod, mod);
+    uint64_t *x = vec, *orig = vec;
+    if (steps > 0)
+        for (uint64_t i = 0; i < n; ++i) x[i] %= mod;
+    for (int i = 0; i < steps; i++) {
+        uint64_t J = 1ull << i;
+        for (uint64_t s = 0; s < n / (2 * J); s++) {
+            uint64_t w = T[J * s];
+            for (uint64_t j = 0; j < J; ++j) {
+                uint64_t l = x[s * J + j], r = x[s * J + j + n / 2];
+                y[2 * s * J + j] = mm64_add(l, r, mod);
+                y[(2 * s + 1) * J + j] = mm64_mul(w, mm64_sub(l, r, mod), mod);
+            }
+        }
+        uint64_t *tp = x;
+        x = y;
+        y = tp;
+    }
+    if (x != orig) memcpy(orig, x, n * sizeof(uint64_t));
+    free(x == orig ? y : x);
+    free(T);
+    return 0;
+}
+
+int orc_stockham_u64_batch(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, uint64_t batches, int nsteps) {
+    for (uint64_t b = 0; b < batches; ++b) {
+        int rc = orc_stockham_u64(vec + b * n, n, root, mod, nsteps);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+/* natural-order inverse: the forward loop with root^-1, then n^-1 */
+int orc_inverse_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod) {
+    if (!is_pow2(n)) return -1;
+    if (n == 1) {
+        vec[0] %= mod;
+        return 0;
+    }
+    int rc = orc_stockham_u64(vec, n, orc_invmod(root % mod, mod), mod, -1);
+    if (rc) return rc;
+    uint64_t ninv = orc_invmod(n % mod, mod);
+    for (uint64_t i = 0; i < n; ++i) vec[i] = mm64_mul(vec[i], ninv, mod);
+    return 0;
+}
+
+int orc_inverse_u64_batch(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, uint64_t batches) {
+    for (uint64_t b = 0; b < batches; ++b) {
+        int rc = orc_inverse_u64(vec + b * n, n, root, mod);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int orc_naive_dft_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t root, uint64_t mod) {
+    for (uint64_t k = 0; k < n; ++k) {
+        uint64_t wk = orc_powmod(root % mod, k, mod), w = 1 % mod, acc = 0;
+        for (uint64_t j = 0; j < n; ++j) {
+            acc = mm64_add(acc, mm64_mul(in[j] % mod, w, mod), mod);
+            w = mm64_mul(w, wk, mod);
+        }
+        out[k] = acc;
+    }
+    return 0;
+}
+
+/* src/ntt/ntt_cpu.cpp:11-42 over Z_mod */
+int orc_ntt_ct_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod) {
+    if (!is_pow2(n)) return -1;
+    uint64_t *T = (uint64_t *)malloc((n / 2 + 1) * sizeof(uint64_t));
+    if (!T) return -2;
+    orc_twiddles_u64(T, n, root % mod, mod);
+    if (n >= 2) bitrev_u64(T, n / 2);
+    for (uint64_t i = 0; i < n; ++i) vec[i] %= mod;
+    for (uint64_t size = n; size >= 2; size /= 2) {
+        uint64_t half = size / 2, k = 0;
+        for (uint64_t i = 0; i < n; i += size) {
+            for (uint64_t j = i; j < i + half; ++j) {
+                uint64_t l = j + half;
+                uint64_t left = vec[j], right = mm64_mul(vec[l], T[k], mod);
+                vec[j] = mm64_add(left, right, mod);
+                vec[l] = mm64_sub(left, right, mod);
+            }
+            k++;
+        }
+    }
+    free(T);
+    return 0;
+}
+
+/* src/ntt/ntt_cpu.cpp:130-164 over Z_mod */
+int orc_intt_gs_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod) {
+    if (!is_pow2(n)) return -1;
+    uint64_t *T = (uint64_t *)malloc((n / 2 + 1) * sizeof(uint64_t));
+    if (!T) return -2;
+    orc_twiddles_u64(T, n, orc_invmod(root % mod, mod), mod);
+    if (n >= 2) bitrev_u64(T, n / 2);
+    for (uint64_t i = 0; i < n; ++i) vec[i] %= mod;
+    for (uint64_t size = 2; size <= n; size *= 2) {
+        uint64_t half = size / 2, k = 0;
+        for (uint64_t i = 0; i < n; i += size) {
+            for (uint64_t j = i; j < i + half; ++j) {
+                uint64_t l = j + half;
+                uint64_t left = vec[j], right = vec[l];
+                vec[j] = mm64_add(left, right, mod);
+                vec[l] = mm64_mul(mm64_sub(left, right, mod), T[k], mod);
+            }
+            k++;
+        }
+    }
+    uint64_t scaler = orc_invmod(n % mod, mod);
+    for (uint64_t i = 0; i < n; ++i) vec[i] = mm64_mul(vec[i], scaler, mod);
+    free(T);
+    return 0;
+}
+
+/* src/ntt/ntt_cpu.cpp:167-174 over Z_mod */
+int orc_pwc_ntt_u64(uint64_t *a, uint64_t *b, uint64_t n, uint64_t root, uint64_t mod) {
+    int rc = orc_ntt_ct_u64(a, n, root, mod);
+    if (!rc) rc = orc_ntt_ct_u64(b, n, root, mod);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < n; ++i) a[i] = mm64_mul(a[i], b[i], mod);
+    return orc_intt_gs_u64(a, n, root, mod);
+}

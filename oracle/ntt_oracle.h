@@ -82,6 +82,17 @@ int orc_ntt_ct_gl(uint64_t *vec, unsigned log2n);
 int orc_intt_gs_gl(uint64_t *vec, unsigned log2n);
 int orc_pwc_ntt_gl(uint64_t *a, uint64_t *b, unsigned log2n);
 
+/* ---- any odd modulus below 2^64, exact (128-bit products) ------------- */
+int orc_twiddles_u64(uint64_t *out, uint64_t n, uint64_t root, uint64_t mod);
+int orc_stockham_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, int nsteps);
+int orc_stockham_u64_batch(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, uint64_t batches, int nsteps);
+int orc_inverse_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod);
+int orc_inverse_u64_batch(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod, uint64_t batches);
+int orc_naive_dft_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t root, uint64_t mod);
+int orc_ntt_ct_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod);
+int orc_intt_gs_u64(uint64_t *vec, uint64_t n, uint64_t root, uint64_t mod);
+int orc_pwc_ntt_u64(uint64_t *a, uint64_t *b, uint64_t n, uint64_t root, uint64_t mod);
+
 /* ---- sampled outputs of the textbook DFT, O(n) per output ------------- */
 int orc_dft_points_gl(const uint64_t *x, unsigned log2n, int inverse, const uint64_t *ks, uint64_t count,
                       uint64_t *out);

@@ -76,6 +76,13 @@ def lib() -> ctypes.CDLL:
         L.orc_ntt_ct_gl.argtypes = [_u64p, _c.c_uint]
         L.orc_intt_gs_gl.argtypes = [_u64p, _c.c_uint]
         L.orc_pwc_ntt_gl.argtypes = [_u64p, _u64p, _c.c_uint]
+        L.orc_twiddles_u64.argtypes = [_u64p, _U64, _U64, _U64]
+        L.orc_stockham_u64_batch.argtypes = [_u64p, _U64, _U64, _U64, _U64, _c.c_int]
+        L.orc_inverse_u64_batch.argtypes = [_u64p, _U64, _U64, _U64, _U64]
+        L.orc_naive_dft_u64.argtypes = [_u64p, _u64p, _U64, _U64, _U64]
+        L.orc_ntt_ct_u64.argtypes = [_u64p, _U64, _U64, _U64]
+        L.orc_intt_gs_u64.argtypes = [_u64p, _U64, _U64, _U64]
+        L.orc_pwc_ntt_u64.argtypes = [_u64p, _u64p, _U64, _U64, _U64]
         L.orc_dft_points_gl.argtypes = [_u64p, _c.c_uint, _c.c_int, _u64p, _U64, _u64p]
         L.orc_dft_points_u32.argtypes = [_u32p, _U64, _U32, _U32, _c.c_int, _u64p, _U64, _u32p]
         _lib = L
@@ -174,6 +181,56 @@ def stockham_idx(x: np.ndarray, n: int, nsteps: int = -1) -> np.ndarray:
     v = np.ascontiguousarray(x, dtype=np.int32).copy()
     _check(lib().orc_stockham_idx(v, n, nsteps), "stockham_idx")
     return v
+
+
+# --------------------------------------------------------------------------
+# any odd modulus below 2^64
+# --------------------------------------------------------------------------
+def _u64(x):
+    return np.ascontiguousarray(x, dtype=np.uint64).copy()
+
+
+def twiddles_u64(n: int, root: int, mod: int) -> np.ndarray:
+    out = np.zeros(max(n // 2, 1), dtype=np.uint64)
+    _check(lib().orc_twiddles_u64(out, n, root, mod), "twiddles_u64")
+    return out[: n // 2]
+
+
+def stockham_u64(x: np.ndarray, n: int, root: int, mod: int, batches: int = 1, nsteps: int = -1) -> np.ndarray:
+    v = _u64(x)
+    _check(lib().orc_stockham_u64_batch(v, n, root, mod, batches, nsteps), "stockham_u64")
+    return v
+
+
+def inverse_u64(x: np.ndarray, n: int, root: int, mod: int, batches: int = 1) -> np.ndarray:
+    v = _u64(x)
+    _check(lib().orc_inverse_u64_batch(v, n, root, mod, batches), "inverse_u64")
+    return v
+
+
+def naive_dft_u64(x: np.ndarray, root: int, mod: int) -> np.ndarray:
+    v = _u64(x)
+    out = np.zeros_like(v)
+    _check(lib().orc_naive_dft_u64(v, out, v.size, root, mod), "naive_dft_u64")
+    return out
+
+
+def ntt_ct_u64(x: np.ndarray, root: int, mod: int) -> np.ndarray:
+    v = _u64(x)
+    _check(lib().orc_ntt_ct_u64(v, v.size, root, mod), "ntt_ct_u64")
+    return v
+
+
+def intt_gs_u64(x: np.ndarray, root: int, mod: int) -> np.ndarray:
+    v = _u64(x)
+    _check(lib().orc_intt_gs_u64(v, v.size, root, mod), "intt_gs_u64")
+    return v
+
+
+def pwc_ntt_u64(a: np.ndarray, b: np.ndarray, root: int, mod: int) -> np.ndarray:
+    va, vb = _u64(a), _u64(b)
+    _check(lib().orc_pwc_ntt_u64(va, vb, va.size, root, mod), "pwc_ntt_u64")
+    return va
 
 
 # --------------------------------------------------------------------------

@@ -22,6 +22,7 @@ STNTT_ENOTFOUND = 6
 
 FIELD_U32 = 0
 FIELD_GOLDILOCKS = 1
+FIELD_U64 = 2
 FORWARD = 0
 INVERSE = 1
 SIGNED_INPUT = 1
@@ -48,6 +49,10 @@ SIGNATURES = [
     ("stntt_ntt_u32", _int, [_vp, _vp, _u32, _u64, _u32, _u32, _int, _u32, _vp]),
     ("stntt_ntt_goldilocks", _int, [_vp, _vp, _u32, _u64, _int, _vp]),
     ("stntt_ntt_goldilocks_ex", _int, [_vp, _vp, _u32, _u64, _int, _u32, _vp]),
+    ("stntt_ntt_u64", _int, [_vp, _vp, _u32, _u64, _u64, _u64, _int, _u32, _vp]),
+    ("stntt_twiddles_u64", _int, [_vp, _u32, _u64, _u64, _int, _vp]),
+    ("stntt_pointwise_u64", _int, [_vp, _vp, _vp, _u64, _u64, _vp]),
+    ("stntt_polymul_u64", _int, [_vp, _vp, _vp, _u32, _u64, _u64, _u64, _vp]),
     ("stntt_pointwise_u32", _int, [_vp, _vp, _vp, _u64, _u32, _vp]),
     ("stntt_pointwise_goldilocks", _int, [_vp, _vp, _vp, _u64, _vp]),
     ("stntt_polymul_u32", _int, [_vp, _vp, _vp, _u32, _u64, _u32, _u32, _u32, _vp]),

@@ -27,6 +27,7 @@
 #include <utility>
 
 #include "field_gl.hpp"
+#include "field_m64.hpp"
 #include "field_u32.hpp"
 
 namespace stntt {
@@ -111,6 +112,7 @@ struct FieldGL {
     }
     // twiddle hi * lo from the two factor tables
     static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &) { return gl_mul(hi, lo); }
+    static __device__ __forceinline__ Tw tw_one(const Rt &) { return 1; }
 };
 
 struct FieldU32 {
@@ -159,6 +161,52 @@ struct FieldU32 {
         uint32_t w = u32_mul(hi.w, lo, rt.p);
         return {w, (uint32_t)(((uint64_t)w << 32) / rt.p)};
     }
+    static __device__ __forceinline__ Tw tw_one(const Rt &rt) {
+        return {1u % rt.p, (uint32_t)(((uint64_t)(1u % rt.p) << 32) / rt.p)};
+    }
+};
+
+// Any odd prime below 2^64 (field_m64.hpp): data and twiddles in Montgomery
+// form inside a transform, converted on the first load and the last store.
+struct FieldM64 {
+    using T = uint64_t;
+    using Tw = uint64_t;  // w R mod p
+    using Rt = M64Rt;
+    struct Roots {
+        Tw w[16];  // omega_r^j (Montgomery form), j < r/2 (r <= 32)
+    };
+    // a product of twiddles is one Montgomery product
+    static constexpr bool RUNNING_TWIDDLES = true;
+
+    static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return m64_add(a, b, rt.p); }
+    static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return m64_sub(a, b, rt.p); }
+    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return m64_mul(a, w, rt.p, rt.pinv); }
+    // a (any form) times b (normal form), keeping a's form
+    static __device__ __forceinline__ T mulg(T a, T b, const Rt &rt) {
+        return m64_mul(a, m64_mul(b, rt.r2, rt.p, rt.pinv), rt.p, rt.pinv);
+    }
+    template <int LR, int DIR>
+    static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
+        Roots r;
+#pragma unroll
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = hi[j << (log_rmax - LR)];
+        return r;
+    }
+    template <int LR, int DIR, int J>
+    static __device__ __forceinline__ T sub_root_mul(T x, T y, const Roots &r, const Rt &rt) {
+        if constexpr (J == 0) return m64_sub(x, y, rt.p);
+        return m64_mul(m64_sub(x, y, rt.p), r.w[J], rt.p, rt.pinv);
+    }
+    // any 64-bit word -> Montgomery form, exactly
+    static __device__ __forceinline__ T load_in(T x, const Rt &rt, uint32_t flags) {
+        return (flags & PASS_FIRST) ? m64_mul(x, rt.r2, rt.p, rt.pinv) : x;
+    }
+    // Montgomery form -> normal form, times n^-1 for the inverse
+    static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
+        return m64_mul(x, (flags & PASS_SCALE) ? rt.ninv : 1, rt.p, rt.pinv);
+    }
+    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) { return m64_mul(hi, lo, rt.p, rt.pinv); }
+    static __device__ __forceinline__ Tw tw_one(const Rt &rt) { return rt.r1; }
 };
 
 // ---------------------------------------------------------------------------

@@ -26,21 +26,20 @@
 
 namespace stntt {
 
-// element at a 32-bit byte offset from a uniform base: lets the compiler use
-// the scalar-base + 32-bit vector-offset addressing form (offsets here stay
-// below 2^31: n <= 2^27 words of at most 8 bytes)
+// element `elem` (a 32-bit index: n <= 2^32) of a uniform base pointer; the
+// byte offset is formed in 64 bits (8-byte words past 2^29 need 33 bits)
 template <class T>
 __device__ __forceinline__ const T &at_off(const T *base, uint32_t elem) {
-    return *(const T *)((const char *)base + elem * (uint32_t)sizeof(T));
+    return *(const T *)((const char *)base + (uint64_t)elem * sizeof(T));
 }
 template <class T>
 __device__ __forceinline__ T &at_off(T *base, uint32_t elem) {
-    return *(T *)((char *)base + elem * (uint32_t)sizeof(T));
+    return *(T *)((char *)base + (uint64_t)elem * sizeof(T));
 }
 
 template <class FP>
 constexpr int tile_logc() {
-    return std::is_same_v<FP, FieldGL> ? 4 : 5;  // 128-byte rows
+    return sizeof(typename FP::T) == 8 ? 4 : 5;  // 128-byte rows
 }
 
 template <int LOGR>

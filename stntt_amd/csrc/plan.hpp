@@ -23,7 +23,7 @@ constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile
 // log2 of the tile size in elements: 64 KiB of LDS per workgroup
 template <class FP>
 constexpr int tile_log() {
-    return std::is_same_v<FP, FieldGL> ? 13 : 14;
+    return sizeof(typename FP::T) == 8 ? 13 : 14;  // 64 KiB of words
 }
 
 template <class FP>

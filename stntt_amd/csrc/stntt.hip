@@ -76,7 +76,6 @@ static uint64_t invmod(uint64_t a, uint64_t m) {
 
 static inline U32Tw shoup(uint32_t w, uint32_t p) { return {w, (uint32_t)(((uint64_t)w << 32) / p)}; }
 
-static uint64_t gl_mul_host(uint64_t a, uint64_t b) { return (uint64_t)((u128)a * b % GL_P); }
 
 // ---------------------------------------------------------------------------
 // twiddle plans: resident factor tables
@@ -99,12 +98,12 @@ struct Plan {
     uint32_t log_rmax = 0;
     FieldGL::Rt rt_gl{};
     FieldU32::Rt rt_u32{};
+    FieldM64::Rt rt_m64{};
     const void *hi() const { return d_tables; }
-    const void *lo() const {
-        size_t esz = key.field == STNTT_FIELD_GOLDILOCKS ? sizeof(uint64_t) : sizeof(U32Tw);
-        return (const char *)d_tables + (esz << log_rmax);
-    }
+    // every field's table entry is 8 bytes (a word, a Shoup pair, a Montgomery word)
+    const void *lo() const { return (const char *)d_tables + (sizeof(uint64_t) << log_rmax); }
 };
+static_assert(sizeof(U32Tw) == 8, "table entries are 8 bytes");
 
 }  // namespace stntt
 
@@ -165,6 +164,38 @@ static int ensure_buffer(void **buf, size_t *cap, size_t bytes) {
     return STNTT_OK;
 }
 
+// Twiddle factor tables generated on the device (the reference builds its
+// table on the host, src/cuda/ntt/implementations/common.cuh:37-41): entry e
+// of hi is wstep^e, entry e of lo is w^e, each by square-and-multiply in the
+// field's twiddle arithmetic (exact, so the words equal a host power loop).
+template <class FP>
+__global__ void gen_tables_kernel(typename FP::Tw *tab, uint32_t rmax, uint32_t nlo, typename FP::Tw w,
+                                  typename FP::Tw wstep, typename FP::Rt rt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rmax + nlo) return;
+    uint32_t e = i < rmax ? i : i - rmax;
+    typename FP::Tw b = i < rmax ? wstep : w;
+    typename FP::Tw r = FP::tw_one(rt);
+    while (e) {
+        if (e & 1) r = FP::twiddle(r, b, rt);
+        b = FP::twiddle(b, b, rt);
+        e >>= 1;
+    }
+    tab[i] = r;
+}
+
+template <class FP>
+static int gen_tables(stntt_ctx *ctx, Plan *plan, typename FP::Tw w, typename FP::Tw wstep, uint64_t rmax,
+                      uint64_t nlo, const typename FP::Rt &rt) {
+    const uint64_t count = rmax + nlo;  // <= 2^14 + 2^(L-14)
+    hipLaunchKernelGGL(gen_tables_kernel<FP>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (typename FP::Tw *)plan->d_tables, (uint32_t)rmax, (uint32_t)nlo, w, wstep, rt);
+    HIP_TRY(hipGetLastError());
+    // transforms may run on other streams: the tables are complete on return
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return STNTT_OK;
+}
+
 static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, uint64_t mod, int dir,
                     const Plan **out) {
     PlanKey key{field, dir, log_n, root, mod};
@@ -177,36 +208,41 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
     plan->key = key;
     uint32_t log_rmax = log_n < MAX_LOGR ? log_n : MAX_LOGR;
     plan->log_rmax = log_rmax;
-    uint64_t n = 1ull << log_n, rmax = 1ull << log_rmax, nlo = n >> log_rmax;
+    const uint64_t n = 1ull << log_n, rmax = 1ull << log_rmax, nlo = n >> log_rmax;
+    const uint64_t p = field == STNTT_FIELD_GOLDILOCKS ? GL_P : mod;
+    // the root for this direction; Goldilocks uses 7^((P-1)/n), 7 being the
+    // smallest generator (what findParams' generator search returns)
+    uint64_t w = field == STNTT_FIELD_GOLDILOCKS ? (log_n ? powmod(7, (GL_P - 1) >> log_n, GL_P) : 1) : root % p;
+    if (dir == STNTT_INVERSE) w = invmod(w, p);
+    const uint64_t wstep = powmod(w, nlo, p);  // w^(n / Rmax): the hi table's ratio
+    plan->bytes = (size_t)(rmax + nlo) * sizeof(uint64_t);
+    HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
+    int rc;
     if (field == STNTT_FIELD_GOLDILOCKS) {
-        uint64_t w = log_n ? powmod(7, (GL_P - 1) >> log_n, GL_P) : 1;
-        if (dir == STNTT_INVERSE) w = invmod(w, GL_P);
-        std::vector<uint64_t> tab(rmax + nlo);
-        uint64_t step = powmod(w, nlo, GL_P), t = 1;
-        for (uint64_t e = 0; e < rmax; ++e, t = gl_mul_host(t, step)) tab[e] = t;
-        t = 1;
-        for (uint64_t e = 0; e < nlo; ++e, t = gl_mul_host(t, w)) tab[rmax + e] = t;
         plan->rt_gl.ninv = invmod(n % GL_P, GL_P);
-        plan->bytes = tab.size() * sizeof(uint64_t);
-        HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
-        HIP_TRY(hipMemcpy(plan->d_tables, tab.data(), plan->bytes, hipMemcpyHostToDevice));
+        rc = gen_tables<FieldGL>(ctx, plan.get(), w, wstep, rmax, nlo, plan->rt_gl);
+    } else if (field == STNTT_FIELD_U32) {
+        const uint32_t p32 = (uint32_t)p;
+        plan->rt_u32.p = p32;
+        plan->rt_u32.bar = (uint32_t)((1ull << 32) / p32);
+        plan->rt_u32.c32 = (uint32_t)((1ull << 32) % p32);
+        plan->rt_u32.ninv = shoup((uint32_t)invmod(n % p32, p32), p32);
+        plan->rt_u32.mu = ~0ull / p32;
+        rc = gen_tables<FieldU32>(ctx, plan.get(), shoup((uint32_t)w, p32), shoup((uint32_t)wstep, p32), rmax, nlo,
+                                  plan->rt_u32);
     } else {
-        uint32_t p = (uint32_t)mod;
-        uint64_t w = root % p;
-        if (dir == STNTT_INVERSE) w = invmod(w, p);
-        std::vector<U32Tw> tab(rmax + nlo);
-        uint64_t step = powmod(w, nlo, p), t = 1 % p;
-        for (uint64_t e = 0; e < rmax; ++e, t = t * step % p) tab[e] = shoup((uint32_t)t, p);
-        t = 1 % p;
-        for (uint64_t e = 0; e < nlo; ++e, t = t * w % p) tab[rmax + e] = shoup((uint32_t)t, p);
-        plan->rt_u32.p = p;
-        plan->rt_u32.bar = (uint32_t)((1ull << 32) / p);
-        plan->rt_u32.c32 = (uint32_t)((1ull << 32) % p);
-        plan->rt_u32.ninv = shoup((uint32_t)invmod(n % p, p), p);
-        plan->rt_u32.mu = ~0ull / p;
-        plan->bytes = tab.size() * sizeof(U32Tw);
-        HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
-        HIP_TRY(hipMemcpy(plan->d_tables, tab.data(), plan->bytes, hipMemcpyHostToDevice));
+        M64Rt &rt = plan->rt_m64;
+        rt.p = p;
+        rt.pinv = m64_neg_inverse(p);
+        rt.r1 = (uint64_t)(((u128)1 << 64) % p);
+        rt.r2 = (uint64_t)((u128)rt.r1 * rt.r1 % p);
+        rt.ninv = invmod(n % p, p);
+        auto mont = [&](uint64_t x) { return (uint64_t)(((u128)x << 64) % p); };
+        rc = gen_tables<FieldM64>(ctx, plan.get(), mont(w), mont(wstep), rmax, nlo, rt);
+    }
+    if (rc) {
+        (void)hipFree(plan->d_tables);
+        return rc;
     }
     *out = plan.get();
     ctx->plans.emplace(key, std::move(plan));
@@ -309,7 +345,8 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     a.log_rmax = plan.log_rmax;
     a.batches = (uint32_t)batches;
     if constexpr (std::is_same_v<FP, FieldGL>) a.rt = plan.rt_gl;
-    else a.rt = plan.rt_u32;
+    else if constexpr (std::is_same_v<FP, FieldU32>) a.rt = plan.rt_u32;
+    else a.rt = plan.rt_m64;
 
     T *src = d_data;
     uint32_t log_j = 0;
@@ -359,8 +396,10 @@ __global__ void expand_twiddles_kernel(typename FP::T *out, const typename FP::T
     typename FP::T x;
     if constexpr (std::is_same_v<FP, FieldGL>) {
         x = gl_canon(gl_mul(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)]));
-    } else {
+    } else if constexpr (std::is_same_v<FP, FieldU32>) {
         x = u32_mul(hi[i >> log_b].w, lo[i & ((1ull << log_b) - 1)], rt.p);
+    } else {  // Montgomery form product, then back to normal form
+        x = m64_mul(m64_mul(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)], rt.p, rt.pinv), 1, rt.p, rt.pinv);
     }
     out[i] = x;
 }
@@ -376,6 +415,20 @@ static int check_u32_params(uint32_t log2n, uint64_t root, uint64_t mod, int dir
     if (log2n >= 1) {
         uint64_t half = powmod(root, 1ull << (log2n - 1), mod);
         if (half != mod - 1)
+            return fail(STNTT_EINVAL, "root %llu is not of order n = 2^%u mod %llu (root^(n/2) != -1)",
+                        (unsigned long long)root, log2n, (unsigned long long)mod);
+        if (dir == STNTT_INVERSE && invmod((1ull << log2n) % mod, mod) == 0)
+            return fail(STNTT_EINVAL, "n is not invertible mod %llu", (unsigned long long)mod);
+    }
+    return STNTT_OK;
+}
+
+static int check_u64_params(uint32_t log2n, uint64_t root, uint64_t mod, int dir) {
+    if (log2n > 32) return fail(STNTT_EINVAL, "log2n %u too large (at most 32)", log2n);
+    if (mod < 3 || !(mod & 1)) return fail(STNTT_EINVAL, "mod %llu must be odd and at least 3", (unsigned long long)mod);
+    if (dir != STNTT_FORWARD && dir != STNTT_INVERSE) return fail(STNTT_EINVAL, "bad direction %d", dir);
+    if (log2n >= 1) {
+        if (powmod(root % mod, 1ull << (log2n - 1), mod) != mod - 1)
             return fail(STNTT_EINVAL, "root %llu is not of order n = 2^%u mod %llu (root^(n/2) != -1)",
                         (unsigned long long)root, log2n, (unsigned long long)mod);
         if (dir == STNTT_INVERSE && invmod((1ull << log2n) % mod, mod) == 0)
@@ -724,6 +777,101 @@ using namespace stntt;
 // C ABI
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
+// 64-bit primality and factoring for the parameter search.  The reference
+// uses trial division (isPrime, uniquePrimeFactors: ntt_util.cpp:51-84),
+// which is exact but takes ~2^32 steps for 64-bit candidates; these give the
+// same answers: Miller-Rabin with the first 12 prime bases is deterministic
+// below 3.3e24 (Sorenson and Webster, Math. Comp. 86, 2017), and factors
+// come from trial division by small primes, then Pollard's rho with Brent's
+// cycle finding, each factor confirmed prime.
+// ---------------------------------------------------------------------------
+static uint64_t mulmod_u64(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((u128)a * b % m); }
+
+static bool is_prime_u64(uint64_t n) {
+    static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    if (n < 2) return false;
+    for (uint64_t q : bases)
+        if (n % q == 0) return n == q;
+    uint64_t d = n - 1;
+    int r = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        ++r;
+    }
+    for (uint64_t a : bases) {
+        uint64_t x = powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool composite = true;
+        for (int i = 1; i < r && composite; ++i) {
+            x = mulmod_u64(x, x, n);
+            if (x == n - 1) composite = false;
+        }
+        if (composite) return false;
+    }
+    return true;
+}
+
+static uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// a nontrivial factor of an odd composite n
+static uint64_t rho_factor(uint64_t n) {
+    for (uint64_t c = 1;; ++c) {
+        auto f = [&](uint64_t x) { return (uint64_t)(((u128)x * x + c) % n); };
+        uint64_t y = 2, x = 2, d = 1, q = 1, ys = 2;
+        const uint64_t m = 128;
+        for (uint64_t len = 1; d == 1; len <<= 1) {
+            x = y;
+            for (uint64_t i = 0; i < len; ++i) y = f(y);
+            for (uint64_t k = 0; k < len && d == 1; k += m) {
+                ys = y;
+                for (uint64_t i = 0; i < m && i < len - k; ++i) {
+                    y = f(y);
+                    q = mulmod_u64(q, x > y ? x - y : y - x, n);
+                }
+                d = gcd_u64(q, n);
+            }
+        }
+        if (d == n) {  // the batched product overshot: step one at a time
+            do {
+                ys = f(ys);
+                d = gcd_u64(x > ys ? x - ys : ys - x, n);
+            } while (d == 1);
+        }
+        if (d != n) return d;
+    }
+}
+
+static void prime_factors(uint64_t n, std::vector<uint64_t> &out) {
+    if (n == 1) return;
+    if (is_prime_u64(n)) {
+        out.push_back(n);
+        return;
+    }
+    uint64_t d = rho_factor(n);
+    prime_factors(d, out);
+    prime_factors(n / d, out);
+}
+
+// distinct prime factors of m, ascending
+static void distinct_prime_factors(uint64_t m, std::vector<uint64_t> &f) {
+    for (uint64_t d = 2; d < 1000 && d * d <= m; ++d)
+        if (m % d == 0) {
+            f.push_back(d);
+            while (m % d == 0) m /= d;
+        }
+    if (m > 1) prime_factors(m, f);
+    std::sort(f.begin(), f.end());
+    f.erase(std::unique(f.begin(), f.end()), f.end());
+}
+
+// ---------------------------------------------------------------------------
 // pointwise products and polynomial multiplication
 // ---------------------------------------------------------------------------
 template <class FP>
@@ -796,30 +944,20 @@ stntt_stream stntt_ctx_stream(stntt_ctx *ctx) { return ctx ? (stntt_stream)ctx->
 int stntt_find_params(uint64_t n, uint64_t min_mod, uint64_t *root, uint64_t *mod) {
     if (!root || !mod) return fail(STNTT_EINVAL, "null output pointer");
     if (n == 0) return fail(STNTT_EINVAL, "n must be positive");
-    // smallest prime p = k n + 1 >= min_mod (trial division)
+    // smallest prime p = k n + 1 >= min_mod (findModulus, ntt_util.cpp:16-24)
     uint64_t k = (min_mod > 1 ? (min_mod - 1 + n - 1) / n : 1);
     if (k < 1) k = 1;
-    auto is_prime = [](uint64_t v) {
-        if (v < 2) return false;
-        for (uint64_t d = 2; (u128)d * d <= v; ++d)
-            if (v % d == 0) return false;
-        return true;
-    };
     uint64_t p;
     for (;; ++k) {
-        if (k > UINT64_MAX / n) return fail(STNTT_EINVAL, "search range exhausted");
+        if (k > (UINT64_MAX - 1) / n) return fail(STNTT_EINVAL, "no prime k n + 1 >= %llu below 2^64",
+                                                  (unsigned long long)min_mod);
         p = k * n + 1;
-        if (is_prime(p)) break;
+        if (is_prime_u64(p)) break;
     }
-    // distinct prime factors of p - 1, then the smallest generator
+    // the smallest generator: c with c^((p-1)/q) != 1 for every prime q | p - 1
+    // (findGenerator / isPrimitiveRoot, ntt_util.cpp:26-49)
     std::vector<uint64_t> f;
-    uint64_t m = p - 1;
-    for (uint64_t d = 2; (u128)d * d <= m; ++d)
-        if (m % d == 0) {
-            f.push_back(d);
-            while (m % d == 0) m /= d;
-        }
-    if (m > 1) f.push_back(m);
+    distinct_prime_factors(p - 1, f);
     uint64_t g = 0;
     for (uint64_t c = 1; c < p && !g; ++c) {
         if (powmod(c, p - 1, p) != 1) continue;
@@ -925,6 +1063,70 @@ int stntt_polymul_goldilocks(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint3
     if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, STNTT_FORWARD, &fwd))) return rc;
     if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, STNTT_INVERSE, &inv))) return rc;
     return polymul_run<FieldGL>(ctx, *fwd, *inv, d_a, d_b, batches, 0, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_ntt_u64(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, uint64_t root, uint64_t mod,
+                  int direction, uint32_t flags, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~(STNTT_BITREV_IN | STNTT_BITREV_OUT))
+        return fail(STNTT_EINVAL, "flags 0x%x (the 64-bit field takes the order flags only)", flags);
+    int rc = check_u64_params(log2n, root, mod, direction);
+    if (rc) return rc;
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U64, log2n, root % mod, mod, direction, &plan))) return rc;
+    return run_transform<FieldM64>(ctx, *plan, d_data, batches, flags, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_pointwise_u64(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_b, uint64_t count, uint64_t mod,
+                        stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (mod < 3 || !(mod & 1)) return fail(STNTT_EINVAL, "mod %llu must be odd and at least 3", (unsigned long long)mod);
+    if (count && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    M64Rt rt{};
+    rt.p = mod;
+    rt.pinv = m64_neg_inverse(mod);
+    rt.r1 = (uint64_t)(((u128)1 << 64) % mod);
+    rt.r2 = (uint64_t)((u128)rt.r1 * rt.r1 % mod);
+    // mulg(a, b) = mont(a, mont(b, R^2)) = a b mod p, exact for any 64-bit
+    // words: each Montgomery product has one operand below p
+    return launch_pointwise<FieldM64>(d_a, d_b, count, rt, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_polymul_u64(stntt_ctx *ctx, uint64_t *d_a, uint64_t *d_b, uint32_t log2n, uint64_t batches, uint64_t root,
+                      uint64_t mod, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    int rc = check_u64_params(log2n, root, mod, STNTT_INVERSE);
+    if (rc) return rc;
+    if (batches && (!d_a || !d_b)) return fail(STNTT_EINVAL, "null data pointer");
+    if (d_a == d_b && batches) return fail(STNTT_EINVAL, "a and b must be distinct buffers");
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *fwd, *inv;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U64, log2n, root % mod, mod, STNTT_FORWARD, &fwd))) return rc;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U64, log2n, root % mod, mod, STNTT_INVERSE, &inv))) return rc;
+    return polymul_run<FieldM64>(ctx, *fwd, *inv, d_a, d_b, batches, 0, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_twiddles_u64(stntt_ctx *ctx, uint32_t log2n, uint64_t root, uint64_t mod, int direction, uint64_t *host_out) {
+    if (!ctx || !host_out) return fail(STNTT_EINVAL, "null argument");
+    int rc = check_u64_params(log2n, root, mod, direction);
+    if (rc) return rc;
+    if (log2n == 0) return STNTT_OK;
+    if ((rc = set_device(ctx))) return rc;
+    const Plan *plan;
+    if ((rc = get_plan(ctx, STNTT_FIELD_U64, log2n, root % mod, mod, direction, &plan))) return rc;
+    uint64_t count = (1ull << log2n) / 2;
+    if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, count * 8))) return rc;
+    hipLaunchKernelGGL(expand_twiddles_kernel<FieldM64>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint64_t *)ctx->staging, (const uint64_t *)plan->hi(),
+                       (const uint64_t *)plan->lo(), log2n - plan->log_rmax, count, plan->rt_m64);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_out, ctx->staging, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return STNTT_OK;
 }
 
 int stntt_twiddles_u32(stntt_ctx *ctx, uint64_t n, uint32_t root, uint32_t mod, int direction, uint32_t *host_out) {
@@ -1264,13 +1466,14 @@ int stntt_server_reset_stats(stntt_server *srv) {
 
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr) {
     if (!ctx) return fail(STNTT_EINVAL, "null context");
-    if (field != STNTT_FIELD_U32 && field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");
+    if (field != STNTT_FIELD_U32 && field != STNTT_FIELD_GOLDILOCKS && field != STNTT_FIELD_U64)
+        return fail(STNTT_EINVAL, "bad field");
     if (npasses == 0) {
         ctx->pass_split.erase({field, log2n});
         return STNTT_OK;
     }
     if (npasses < 2 || npasses > 4 || !logr) return fail(STNTT_EINVAL, "pass count %u outside [2, 4]", npasses);
-    const int logc = field == STNTT_FIELD_GOLDILOCKS ? tile_logc<FieldGL>() : tile_logc<FieldU32>();
+    const int logc = field == STNTT_FIELD_U32 ? tile_logc<FieldU32>() : tile_logc<FieldGL>();
     uint32_t sum = 0;
     std::vector<int> v;
     for (uint32_t i = 0; i < npasses; ++i) {

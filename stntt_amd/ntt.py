@@ -123,6 +123,16 @@ class Context:
                                             _lib.INVERSE if inverse else _lib.FORWARD,
                                             _order_flags(bitrev_in, bitrev_out), stream))
 
+    def ntt_u64(self, buf: "DeviceBuffer | int", log2n: int, batches: int, root: int, mod: int,
+                inverse: bool = False, stream: int | None = None, bitrev_in: bool = False,
+                bitrev_out: bool = False) -> None:
+        """Any odd prime mod < 2^64 with a root of order 2^log2n (e.g. from
+        find_params with a 64-bit min_mod); in place, any input word."""
+        ptr = buf.ptr if isinstance(buf, DeviceBuffer) else buf
+        check(lib().stntt_ntt_u64(self.handle, ptr, log2n, batches, root, mod,
+                                  _lib.INVERSE if inverse else _lib.FORWARD, _order_flags(bitrev_in, bitrev_out),
+                                  stream))
+
     # ---- pointwise products and polynomial multiplication ----------------
     def pointwise_u32(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int, mod: int,
                       stream: int | None = None) -> None:
@@ -143,16 +153,32 @@ class Context:
                            stream: int | None = None) -> None:
         check(lib().stntt_polymul_goldilocks(self.handle, a.ptr, b.ptr, log2n, batches, stream))
 
+    def pointwise_u64(self, a: "DeviceBuffer", b: "DeviceBuffer", count: int, mod: int,
+                      stream: int | None = None) -> None:
+        check(lib().stntt_pointwise_u64(self.handle, a.ptr, b.ptr, count, mod, stream))
+
+    def polymul_u64(self, a: "DeviceBuffer", b: "DeviceBuffer", log2n: int, batches: int, root: int, mod: int,
+                    stream: int | None = None) -> None:
+        check(lib().stntt_polymul_u64(self.handle, a.ptr, b.ptr, log2n, batches, root, mod, stream))
+
     # ---- twiddle tables in the reference layout ---------------------------
     def twiddles_u32(self, n: int, root: int, mod: int, inverse: bool = False) -> np.ndarray:
         out = np.zeros(max(n // 2, 1), dtype=np.uint32)
         check(lib().stntt_twiddles_u32(self.handle, n, root, mod, int(inverse), out.ctypes.data))
         return out[: n // 2]
 
+    def twiddles_u64(self, log2n: int, root: int, mod: int, inverse: bool = False) -> np.ndarray:
+        out = np.zeros(max((1 << log2n) // 2, 1), dtype=np.uint64)
+        check(lib().stntt_twiddles_u64(self.handle, log2n, root, mod, int(inverse), out.ctypes.data))
+        return out[: (1 << log2n) // 2]
+
     def twiddles_goldilocks(self, log2n: int, inverse: bool = False) -> np.ndarray:
         out = np.zeros(max((1 << log2n) // 2, 1), dtype=np.uint64)
         check(lib().stntt_twiddles_goldilocks(self.handle, log2n, int(inverse), out.ctypes.data))
         return out[: (1 << log2n) // 2]
+
+
+_FIELDS = {"u32": _lib.FIELD_U32, "goldilocks": _lib.FIELD_GOLDILOCKS, "u64": _lib.FIELD_U64}
 
 
 def _order_flags(bitrev_in: bool, bitrev_out: bool) -> int:
@@ -410,7 +436,7 @@ class Batch:
         arr = (_lib.Desc * max(len(descs), 1))()
         for i, (field, inverse, log2n, buf, root, mod) in enumerate(descs):
             ptr = buf.ptr if isinstance(buf, DeviceBuffer) else buf
-            arr[i] = _lib.Desc(_lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32,
+            arr[i] = _lib.Desc(_FIELDS[field],
                                _lib.INVERSE if inverse else _lib.FORWARD, log2n, 0,
                                root or 0, mod or 0, ptr, ptr)
         h = ctypes.c_void_p()
@@ -444,7 +470,7 @@ class Batch:
 
 def set_pass_split(ctx: Context, field: str, log2n: int, logr: list[int] | None) -> None:
     """Install an autotuned stages-per-pass split for the multi-pass path."""
-    f = _lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32
+    f = _FIELDS[field]
     arr = (ctypes.c_uint32 * max(len(logr or []), 1))(*(logr or []))
     check(lib().stntt_set_pass_split(ctx.handle, f, log2n, len(logr or []), arr))
 
@@ -496,7 +522,7 @@ class Server:
 
     def submit(self, field: str, log2n: int, data: PinnedArray, inverse: bool = False, root: int = 0,
                mod: int = 0, out: PinnedArray | None = None) -> int:
-        req = _lib.Request(_lib.FIELD_GOLDILOCKS if field == "goldilocks" else _lib.FIELD_U32,
+        req = _lib.Request(_FIELDS[field],
                            _lib.INVERSE if inverse else _lib.FORWARD, log2n, 0, root, mod, data.ptr,
                            (out or data).ptr)
         rid = ctypes.c_uint64()

@@ -1,11 +1,12 @@
 // field_test.cpp -- host-side check of the device field arithmetic
-// (stntt_amd/csrc/field_gl.hpp, field_u32.hpp) against 128-bit integer
+// (stntt_amd/csrc/field_gl.hpp, field_u32.hpp, field_m64.hpp) against 128-bit integer
 // arithmetic on edge values and random values.  Runs on the CPU.
 #include <cstdio>
 #include <random>
 #include <vector>
 
 #include "../../stntt_amd/csrc/field_gl.hpp"
+#include "../../stntt_amd/csrc/field_m64.hpp"
 #include "../../stntt_amd/csrc/field_u32.hpp"
 
 using namespace stntt;
@@ -94,6 +95,35 @@ int main() {
                 CHECK(u32_mul_any(a, b, p, ~0ull / p) == (uint32_t)((uint64_t)a * b % p), "u32 mul_any %u * %u mod %u",
                       a, b, p);
             }
+        }
+    }
+    // 64-bit Montgomery field: primes below 2^32, between 2^32 and 2^63,
+    // above 2^63, Goldilocks and the largest 64-bit prime
+    for (uint64_t p : std::vector<uint64_t>{65537ull, 4294967291ull, 1099511627791ull, 0x3FFFFFFFFFFFFFFDull,
+                                            9223372036854775837ull, GL_P, 18446744073709551557ull}) {
+        uint64_t pinv = m64_neg_inverse(p);
+        CHECK(p * pinv == (uint64_t)0 - 1, "m64 -p^-1 for %llu", (unsigned long long)p);
+        uint64_t r1 = (uint64_t)(((u128)1 << 64) % p), r2 = (uint64_t)((u128)r1 * r1 % p);
+        std::vector<uint64_t> xs = {0, 1, 2, p - 1, p - 2, p / 2, p / 2 + 1, 0xFFFFFFFFull, 1ull << 32};
+        for (int i = 0; i < 300; ++i) xs.push_back(rng() % p);
+        for (uint64_t &x : xs) x %= p;  // add and sub take canonical words
+        std::vector<uint64_t> raw = {p, p + 1, ~0ull, ~0ull - 1, 1ull << 63};
+        for (int i = 0; i < 100; ++i) raw.push_back(rng());
+        auto mont = [&](uint64_t x) { return (uint64_t)(((u128)(x % p) << 64) % p); };
+        for (uint64_t a : xs) {
+            for (uint64_t b : xs) {
+                CHECK(m64_add(a, b, p) == (uint64_t)(((u128)a + b) % p), "m64 add");
+                CHECK(m64_sub(a, b, p) == (uint64_t)(((u128)a + p - b) % p), "m64 sub");
+                // mont(a R, b) = a b: a product in the transform's convention
+                CHECK(m64_mul(mont(a), b, p, pinv) == (uint64_t)((u128)a * b % p), "m64 mul %llu %llu mod %llu",
+                      (unsigned long long)a, (unsigned long long)b, (unsigned long long)p);
+            }
+        }
+        // any 64-bit word converts exactly: mont(x, R^2) = x R mod p; back with mont(., 1)
+        for (uint64_t x : raw) {
+            uint64_t xm = m64_mul(x, r2, p, pinv);
+            CHECK(xm == mont(x), "m64 to-Montgomery of %llu mod %llu", (unsigned long long)x, (unsigned long long)p);
+            CHECK(m64_mul(xm, 1, p, pinv) == x % p, "m64 from-Montgomery");
         }
     }
     // 2^39 is a primitive 64th root of unity (omega_64 for generator 7)
