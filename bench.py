@@ -198,6 +198,25 @@ def bench_orders_polymul(S, ctx, steps: int) -> dict:
     return out
 
 
+def bench_field64(S, ctx, steps: int) -> dict:
+    """The 64-bit prime field (Montgomery arithmetic) at 2^24 forward, for a
+    prime above 2^63 and for P itself beside the special-form Goldilocks
+    kernels (same words, so the ratio is the price of generality)."""
+    out = {}
+    L = 24
+    n = 1 << L
+    x = np.random.default_rng(9).integers(0, 2**64, size=n, dtype=np.uint64)
+    buf = ctx.upload(x)
+    for name, min_mod in (("p_above_2^63", 2**63 + 1), ("p_goldilocks", P)):
+        root, mod = S.find_params(n, min_mod)
+        t = _time_steps(ctx, lambda: ctx.ntt_u64(buf, L, 1, root, mod), steps)
+        out[name] = {"mod": mod, "forward_ms": round(t * 1e3, 4), "gelem_s": round(n / t / 1e9, 3)}
+    t = _time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1), steps)
+    out["goldilocks_kernels_forward_ms"] = round(t * 1e3, 4)
+    buf.free()
+    return out
+
+
 def bench_batched(S, ctx, count: int, reps: int) -> dict:
     """4096 transforms, log2 n ~ U{10..18}, field and direction 50/50: one
     persistent launch (graph replay) versus per-transform plain launches."""
@@ -408,6 +427,8 @@ def main() -> int:
         out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
         log("[bench] element orders and polynomial multiply")
         out["orders_polymul"] = bench_orders_polymul(S, ctx, 20)
+        log("[bench] 64-bit prime field")
+        out["field64_2^24"] = bench_field64(S, ctx, 20)
         log(f"[bench] batched mixed workload ({args.batch_count} transforms)")
         out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10)
         log("[bench] serving stream")

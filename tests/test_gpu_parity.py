@@ -193,6 +193,26 @@ def test_goldilocks_large(ctx, L):
         assert np.array_equal(X, O.stockham_gl(x, L))
 
 
+def test_goldilocks_beyond_4gib(ctx):
+    """2^30 Goldilocks words (8 GiB): tile offsets past 2^32 bytes.  Sampled
+    outputs against the O(n)-per-point DFT, and the round trip."""
+    L = 30
+    n = 1 << L
+    print("goldilocks 2^30: building input", flush=True)
+    x = np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+    buf = ctx.upload(x)
+    ctx.ntt_goldilocks(buf, L, 1)
+    ks = np.array([1, n // 2 + 3, n - 1], dtype=np.uint64)
+    X = buf.copy_to(np.empty_like(x))
+    print("goldilocks 2^30: sampled points", flush=True)
+    assert np.array_equal(X[ks], O.dft_points_gl(x, L, ks))
+    del X
+    ctx.ntt_goldilocks(buf, L, 1, inverse=True)
+    y = buf.copy_to(np.empty_like(x))
+    assert np.array_equal(y, x % np.uint64(P))
+    buf.free()
+
+
 @pytest.mark.parametrize("L", [23, 24, 27])
 def test_u32_large(ctx, L):
     n = 1 << L
