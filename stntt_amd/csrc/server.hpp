@@ -4,18 +4,23 @@
 // one transform at a time: src/bench.cu:26-55, src/main.cu:37-60).
 //
 // A server owns W worker contexts (one HIP stream, scratch pool and resident
-// twiddle plans each) and a device-buffer cache in power-of-two size classes.
-// Clients submit requests (field, direction, log2 n, pinned host in/out) from
-// any thread; a server thread
+// twiddle plans each), one upload and one download stream, and a
+// device-buffer cache in power-of-two size classes.  Clients submit requests
+// (field, direction, log2 n, pinned host in/out) from any thread; a server
+// thread
 //   1. admits pending requests in arrival order while the device bytes held
 //      by admitted requests stay within the HBM budget;
-//   2. launches each admission round on the least-loaded worker stream: the
-//      round's small transforms (log2 n <= SERVER_BATCH_MAX_LOG) go into one
-//      persistent descriptor-queue launch, larger ones run the multi-pass
-//      path; uploads, kernels and downloads are asynchronous on that stream,
-//      so rounds on different workers overlap on the device;
-//   3. retires rounds whose completion event has fired, returns their bytes
-//      to the budget and their buffers to the cache, and wakes waiters.
+//   2. launches each admission round: its uploads go on the upload stream in
+//      admission order (one transfer at a time at full PCIe bandwidth, not
+//      shared among workers); a free worker stream waits for them and runs
+//      the round's small transforms (log2 n <= SERVER_BATCH_MAX_LOG) as one
+//      persistent descriptor-queue launch, then each larger one through the
+//      multi-pass path; the download stream takes each result back as soon
+//      as its own kernel is done.  Uploads, kernels and downloads of
+//      different rounds overlap;
+//   3. retires each group of a round (the batched small requests, or one
+//      large request) when its download event has fired, returns its bytes
+//      to the budget and its buffers to the cache, and wakes waiters.
 // Requests arriving while others run are admitted as soon as budget frees
 // up: continuous batching, not a queue drained one request at a time.
 #pragma once
@@ -64,16 +69,25 @@ struct ServerWorker {
     int inflight = 0;
 };
 
+// requests of a round that complete together: their downloads end at ev
+struct ServerGroup {
+    hipEvent_t ev = nullptr;
+    std::vector<uint64_t> ids;
+    bool done = false;
+};
+
 struct ServerRound {
     int worker;
-    hipEvent_t done_ev;
-    std::vector<uint64_t> ids;
+    std::vector<ServerGroup> groups;
+    std::vector<hipEvent_t> aux;  // ordering events, recycled when the round retires
+    size_t pending = 0;           // groups not yet retired
 };
 
 }  // namespace stntt
 
 struct stntt_server {
     int device = 0;
+    hipStream_t up = nullptr, down = nullptr;  // host->device and device->host copies
     uint64_t budget = 0;
     uint64_t used = 0;
     std::vector<stntt::ServerWorker> workers;

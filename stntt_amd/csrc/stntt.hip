@@ -624,46 +624,94 @@ static void *server_get_buffer(stntt_server *S, size_t bytes) {
 
 // Launch one admission round on worker w.  Runs on the server thread with
 // the lock released; the requests it touches are owned by the round.
-static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest *> &round, hipEvent_t ev) {
+static hipEvent_t server_event(stntt_server *S) {
+    if (!S->free_events.empty()) {
+        hipEvent_t e = S->free_events.back();
+        S->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Called by the server thread only (it alone touches the event pool).
+static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest *> &round, ServerRound &rd) {
     ServerWorker &W = S->workers[w];
     hipStream_t s = W.stream;
-    uint32_t nb = 0;
+    auto ev = [&](std::vector<hipEvent_t> &into) -> hipEvent_t {
+        hipEvent_t e = server_event(S);
+        if (e) into.push_back(e);
+        return e;
+    };
+    // 1. uploads in admission order on the upload stream; the worker waits
     for (ServerRequest *r : round) {
         const stntt_request &q = r->req;
-        const size_t bytes = (size_t)elem_bytes(q.field) << q.log2n;
-        HIP_TRY(hipMemcpyAsync(r->d_buf, q.host_in, bytes, hipMemcpyHostToDevice, s));
-        if (q.log2n <= SERVER_BATCH_MAX_LOG) {
-            BatchTask &t = W.h_tasks[nb++];
-            t = BatchTask{};
-            const Plan *plan;
-            int rc;
-            if (q.field == STNTT_FIELD_U32) {
-                if ((rc = get_plan(W.ctx, STNTT_FIELD_U32, q.log2n, q.root % q.mod, q.mod, q.direction, &plan))) return rc;
-                t.rt32 = plan->rt_u32;
-            } else {
-                if ((rc = get_plan(W.ctx, STNTT_FIELD_GOLDILOCKS, q.log2n, 0, GL_P, q.direction, &plan))) return rc;
-                t.rtgl = plan->rt_gl;
-            }
-            t.src = r->d_buf;
-            t.dst = r->d_buf;
-            t.hi = plan->hi();
-            t.lo = plan->lo();
-            t.field = q.field;
-            t.dir = q.direction;
-            t.log_n = q.log2n;
-            t.log_rmax = plan->log_rmax;
-            t.flags = PASS_FIRST | PASS_LAST | (q.direction == STNTT_INVERSE ? PASS_SCALE : 0) |
-                      ((q.field == STNTT_FIELD_U32 && (q.flags & STNTT_SIGNED_INPUT)) ? PASS_SIGNED : 0);
-            default_batch_split(q.log2n, batch_global_path(q.field, q.log2n), &t.nsub, &t.radix_logs);
-        }
+        HIP_TRY(hipMemcpyAsync(r->d_buf, q.host_in, (size_t)elem_bytes(q.field) << q.log2n, hipMemcpyHostToDevice,
+                               S->up));
     }
+    hipEvent_t ev_in = ev(rd.aux);
+    if (!ev_in) return fail(STNTT_EDEVICE, "hipEventCreate failed");
+    HIP_TRY(hipEventRecord(ev_in, S->up));
+    HIP_TRY(hipStreamWaitEvent(s, ev_in, 0));
+    // 2. the small transforms in one persistent launch, then their downloads
+    uint32_t nb = 0;
+    ServerGroup small;
+    for (ServerRequest *r : round) {
+        const stntt_request &q = r->req;
+        if (q.log2n > SERVER_BATCH_MAX_LOG) continue;
+        BatchTask &t = W.h_tasks[nb++];
+        t = BatchTask{};
+        const Plan *plan;
+        int rc;
+        if (q.field == STNTT_FIELD_U32) {
+            if ((rc = get_plan(W.ctx, STNTT_FIELD_U32, q.log2n, q.root % q.mod, q.mod, q.direction, &plan))) return rc;
+            t.rt32 = plan->rt_u32;
+        } else {
+            if ((rc = get_plan(W.ctx, STNTT_FIELD_GOLDILOCKS, q.log2n, 0, GL_P, q.direction, &plan))) return rc;
+            t.rtgl = plan->rt_gl;
+        }
+        t.src = r->d_buf;
+        t.dst = r->d_buf;
+        t.hi = plan->hi();
+        t.lo = plan->lo();
+        t.field = q.field;
+        t.dir = q.direction;
+        t.log_n = q.log2n;
+        t.log_rmax = plan->log_rmax;
+        t.flags = PASS_FIRST | PASS_LAST | (q.direction == STNTT_INVERSE ? PASS_SCALE : 0) |
+                  ((q.field == STNTT_FIELD_U32 && (q.flags & STNTT_SIGNED_INPUT)) ? PASS_SIGNED : 0);
+        default_batch_split(q.log2n, batch_global_path(q.field, q.log2n), &t.nsub, &t.radix_logs);
+        small.ids.push_back(r->id);
+    }
+    auto download = [&](ServerGroup &g, const std::vector<ServerRequest *> &reqs) -> int {
+        hipEvent_t ev_c = ev(rd.aux);
+        if (!ev_c) return fail(STNTT_EDEVICE, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(ev_c, s));
+        HIP_TRY(hipStreamWaitEvent(S->down, ev_c, 0));
+        for (ServerRequest *r : reqs)
+            HIP_TRY(hipMemcpyAsync(r->req.host_out, r->d_buf, (size_t)elem_bytes(r->req.field) << r->req.log2n,
+                                   hipMemcpyDeviceToHost, S->down));
+        g.ev = server_event(S);
+        if (!g.ev) return fail(STNTT_EDEVICE, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(g.ev, S->down));
+        rd.groups.push_back(std::move(g));
+        rd.pending++;
+        return STNTT_OK;
+    };
     if (nb) {
         HIP_TRY(hipMemcpyAsync(W.d_tasks, W.h_tasks, nb * sizeof(BatchTask), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, W.d_counter);
         hipLaunchKernelGGL(ntt_batch_kernel, dim3(std::min<uint32_t>(nb, SERVER_BATCH_GRID)), dim3(BATCH_THREADS), 0, s,
                            W.d_tasks, nb, W.d_counter, W.d_scratch, (uint64_t)1 << SERVER_BATCH_MAX_LOG);
         HIP_TRY(hipGetLastError());
+        std::vector<ServerRequest *> reqs;
+        for (ServerRequest *r : round)
+            if (r->req.log2n <= SERVER_BATCH_MAX_LOG) reqs.push_back(r);
+        int rc = download(small, reqs);
+        if (rc) return rc;
     }
+    // 3. each large transform, its download right behind it
     for (ServerRequest *r : round) {
         const stntt_request &q = r->req;
         if (q.log2n <= SERVER_BATCH_MAX_LOG) continue;
@@ -671,12 +719,10 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
                      ? stntt_ntt_goldilocks(W.ctx, (uint64_t *)r->d_buf, q.log2n, 1, q.direction, s)
                      : stntt_ntt_u32(W.ctx, (uint32_t *)r->d_buf, q.log2n, 1, q.root, q.mod, q.direction, q.flags, s);
         if (rc) return rc;
+        ServerGroup g;
+        g.ids.push_back(r->id);
+        if ((rc = download(g, {r}))) return rc;
     }
-    for (ServerRequest *r : round) {
-        const stntt_request &q = r->req;
-        HIP_TRY(hipMemcpyAsync(q.host_out, r->d_buf, (size_t)elem_bytes(q.field) << q.log2n, hipMemcpyDeviceToHost, s));
-    }
-    HIP_TRY(hipEventRecord(ev, s));
     return STNTT_OK;
 }
 
@@ -684,29 +730,35 @@ static void server_loop(stntt_server *S) {
     (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        // 1. retire finished rounds
+        // 1. retire finished groups, and rounds whose groups all finished
         bool progressed = false;
         for (auto it = S->rounds.begin(); it != S->rounds.end();) {
-            if (hipEventQuery(it->done_ev) != hipSuccess) {
+            for (ServerGroup &g : it->groups) {
+                if (g.done || hipEventQuery(g.ev) != hipSuccess) continue;
+                const auto now = Clock::now();
+                for (uint64_t id : g.ids) {
+                    ServerRequest &r = S->reqs[id];
+                    r.done = true;
+                    r.latency_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
+                    S->latencies.push_back(r.latency_ms);
+                    S->used -= r.cost;
+                    S->cache[r.d_bytes].push_back(r.d_buf);
+                    r.d_buf = nullptr;
+                    S->completed++;
+                }
+                S->t_last = now;
+                g.done = true;
+                it->pending--;
+                S->free_events.push_back(g.ev);
+                progressed = true;
+            }
+            if (it->pending == 0) {
+                S->workers[it->worker].inflight--;
+                for (hipEvent_t e : it->aux) S->free_events.push_back(e);
+                it = S->rounds.erase(it);
+            } else {
                 ++it;
-                continue;
             }
-            const auto now = Clock::now();
-            for (uint64_t id : it->ids) {
-                ServerRequest &r = S->reqs[id];
-                r.done = true;
-                r.latency_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
-                S->latencies.push_back(r.latency_ms);
-                S->used -= r.cost;
-                S->cache[r.d_bytes].push_back(r.d_buf);
-                r.d_buf = nullptr;
-                S->completed++;
-            }
-            S->t_last = now;
-            S->workers[it->worker].inflight--;
-            S->free_events.push_back(it->done_ev);
-            it = S->rounds.erase(it);
-            progressed = true;
         }
         if (progressed) S->cv_done.notify_all();
         if (S->stop && S->pending.empty() && S->rounds.empty()) break;
@@ -733,23 +785,17 @@ static void server_loop(stntt_server *S) {
             }
         }
         if (!round.empty()) {
-            hipEvent_t ev;
-            if (!S->free_events.empty()) {
-                ev = S->free_events.back();
-                S->free_events.pop_back();
-            } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-                ev = nullptr;
-            }
             S->workers[w].inflight++;
             S->rounds_launched++;
             for (ServerRequest *r : round) S->batched_requests += r->req.log2n <= SERVER_BATCH_MAX_LOG ? 1 : 0;
-            ServerRound rd{w, ev, {}};
-            for (ServerRequest *r : round) rd.ids.push_back(r->id);
+            ServerRound rd{w, {}, {}, 0};
             lk.unlock();
-            int rc = ev ? server_launch_round(S, w, round, ev) : STNTT_EDEVICE;
+            int rc = server_launch_round(S, w, round, rd);
             lk.lock();
-            if (rc) {  // fail the round's requests; nothing of it is in flight
-                if (ev) (void)hipStreamSynchronize(S->workers[w].stream);
+            if (rc) {  // fail the round's requests once nothing of it is in flight
+                (void)hipStreamSynchronize(S->up);
+                (void)hipStreamSynchronize(S->workers[w].stream);
+                (void)hipStreamSynchronize(S->down);
                 for (ServerRequest *r : round) {
                     r->done = true;
                     r->status = rc;
@@ -758,7 +804,8 @@ static void server_loop(stntt_server *S) {
                     r->d_buf = nullptr;
                 }
                 S->workers[w].inflight--;
-                if (ev) S->free_events.push_back(ev);
+                for (ServerGroup &g : rd.groups) S->free_events.push_back(g.ev);
+                for (hipEvent_t e : rd.aux) S->free_events.push_back(e);
                 S->cv_done.notify_all();
             } else {
                 S->rounds.push_back(std::move(rd));
@@ -1317,6 +1364,9 @@ int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers,
     auto S = std::make_unique<stntt_server>();
     S->device = device;
     S->budget = hbm_budget_bytes - reserve;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&S->up, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&S->down, hipStreamNonBlocking));
     for (uint32_t i = 0; i < workers; ++i) {
         ServerWorker W;
         int rc = stntt_ctx_create(device, &W.ctx);
@@ -1346,6 +1396,8 @@ int stntt_server_destroy(stntt_server *srv) {
     srv->cv_work.notify_all();
     if (srv->thread.joinable()) srv->thread.join();
     (void)hipSetDevice(srv->device);
+    if (srv->up) (void)hipStreamSynchronize(srv->up);
+    if (srv->down) (void)hipStreamSynchronize(srv->down);
     for (auto &W : srv->workers) {
         (void)hipStreamSynchronize(W.stream);
         (void)hipHostFree(W.h_tasks);
@@ -1359,6 +1411,8 @@ int stntt_server_destroy(stntt_server *srv) {
     for (auto &kv : srv->reqs)
         if (kv.second.d_buf) (void)hipFree(kv.second.d_buf);
     for (hipEvent_t e : srv->free_events) (void)hipEventDestroy(e);
+    if (srv->up) (void)hipStreamDestroy(srv->up);
+    if (srv->down) (void)hipStreamDestroy(srv->down);
     delete srv;
     return STNTT_OK;
 }
