@@ -161,11 +161,16 @@ int stntt_auto_polymul_i32(stntt_ctx *ctx, int32_t *host_a, int32_t *host_b, uin
                            uint32_t mod, float *kernel_ms);
 
 /* ---- batched mixed transforms -------------------------------------------
-   Thousands of independent transforms of mixed size, field and direction in
-   one persistent launch: resident workgroups pull descriptors from a device
-   work queue (largest first) and each runs its transform to completion.  No
-   counterpart in the reference, whose autoNtt<n> runs one transform (or one
-   same-size batch) per call (src/cuda/ntt/ntt_kernel.cuh:152-201). */
+   Thousands of independent transforms of mixed size (log2n <= 24), field and
+   direction.  Transforms that fit a workgroup's LDS (32 KiB: 2^12
+   Goldilocks, 2^13 32-bit words) run in one persistent launch: resident
+   workgroups pull descriptors from a device work queue (largest first) and
+   each runs its transform to completion.  Larger ones are grouped by
+   (field, size, direction, parameters) and each group runs the multi-pass
+   tile kernels over tables of its descriptors' buffers, full-chip launches
+   with a scratch owned by the batch.  No counterpart in the reference, whose
+   autoNtt<n> runs one transform (or one same-size batch) per call
+   (src/cuda/ntt/ntt_kernel.cuh:152-201). */
 typedef struct stntt_desc {
     uint32_t field;     /* stntt_field */
     uint32_t direction; /* stntt_direction */
@@ -180,7 +185,8 @@ typedef struct stntt_batch stntt_batch;
 /* Validates every descriptor, makes the twiddle plans resident, orders the
    queue and uploads it.  The descriptors' buffers must stay valid. */
 int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out);
-/* one persistent launch (preceded by a one-thread queue reset kernel) */
+/* the persistent launch (preceded by a one-thread queue reset kernel), then
+   the groups' pass launches, on one stream */
 int stntt_batch_run(stntt_batch *b, stntt_stream stream);
 /* capture the launch into a HIP graph once; later runs replay it */
 int stntt_batch_capture(stntt_batch *b, stntt_stream stream);

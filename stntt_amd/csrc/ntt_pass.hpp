@@ -250,6 +250,10 @@ struct PassArgs {
     // first pass only: when set, input word i is multiplied by mul_in[i]
     // before the transform (a pointwise product fused into the load)
     const typename FP::T *mul_in;
+    // gathered batches: when set, transform b reads from in_tab[b] (first
+    // pass) / writes to out_tab[b] (last pass) instead of in + b n / out + b n
+    const typename FP::T *const *in_tab;
+    typename FP::T *const *out_tab;
 };
 
 // w^e for e < n through the two factor tables
@@ -363,7 +367,10 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
         T x = 0;
         if ((uint32_t)c < ncols) {
             const uint64_t gi = in_base + c * cs_in + m * ms_in;
-            x = FP::load_in(a.in[gi], a.rt, a.flags);
+            T raw;
+            if constexpr (SINGLE) raw = a.in_tab ? a.in_tab[(uint64_t)blockIdx.x * C + c][m] : a.in[gi];
+            else raw = a.in[gi];
+            x = FP::load_in(raw, a.rt, a.flags);
             if (a.mul_in) x = FP::mulg(x, a.mul_in[gi], a.rt);
         }
         // bit-reversed input: word m of a transform is DFT input bitrev(m)
@@ -402,6 +409,12 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
             if (e) x = apply_twiddle<FP>(x, e, a);
         }
         if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
+        if constexpr (SINGLE) {
+            if (a.out_tab) {
+                a.out_tab[(uint64_t)blockIdx.x * C + c][k] = x;
+                continue;
+            }
+        }
         a.out[out_base + c * cs_out + k * ks_out] = x;
     }
 }

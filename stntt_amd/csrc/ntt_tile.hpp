@@ -95,8 +95,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     const uint32_t stride_in = 1u << (logn - LOGR);  // n / R
     const uint32_t log_b = logn - a.log_rmax;        // w^e = hi[e >> log_b] lo[e & bmask]
     const uint32_t bmask = (1u << log_b) - 1;
-    const T *in = a.in + base;
-    T *out = a.out + base;
+    const T *in = a.in_tab ? a.in_tab[b] : a.in + base;
+    T *out = a.out_tab ? a.out_tab[b] : a.out + base;
     // all columns of the tile share s, hence the pass twiddles w^(s J k)
     const uint32_t sJ = J1 ? 0u : ((t0 >> a.log_j) << a.log_j);
 

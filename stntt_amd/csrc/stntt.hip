@@ -105,6 +105,20 @@ struct Plan {
 };
 static_assert(sizeof(U32Tw) == 8, "table entries are 8 bytes");
 
+// Batched transforms above the LDS-resident size, grouped by everything that
+// selects a plan and flags: each group runs the multi-pass tile kernels over
+// tables of its descriptors' buffers (a full-chip launch per pass), instead
+// of one workgroup streaming each transform through global scratch.
+struct BatchGroup {
+    int field = 0, dir = 0;
+    uint32_t log2n = 0, flags = 0;
+    uint64_t root = 0, mod = 0;
+    const Plan *plan = nullptr;
+    uint32_t count = 0;
+    void **d_src = nullptr;  // device table of source pointers
+    void **d_dst = nullptr;  // device table of destination pointers
+};
+
 }  // namespace stntt
 
 struct stntt_ctx {
@@ -131,8 +145,10 @@ struct stntt_ctx {
 struct stntt_batch {
     stntt_ctx *ctx = nullptr;
     std::vector<stntt_desc> descs;        // as given (for the plain path)
-    stntt::BatchTask *d_tasks = nullptr;  // queue, largest first
-    uint32_t count = 0;
+    stntt::BatchTask *d_tasks = nullptr;  // persistent-kernel queue, largest first
+    uint32_t count = 0;                   // tasks in the queue
+    std::vector<stntt::BatchGroup> groups;
+    void *d_group_scratch = nullptr;  // ping-pong of the largest group
     uint32_t *d_counter = nullptr;
     uint64_t *d_scratch = nullptr;
     uint64_t scratch_stride = 0;  // 64-bit words per workgroup
@@ -307,10 +323,16 @@ __global__ void copy_kernel(const typename FP::T *in, typename FP::T *out, uint6
 // Runs the pass schedule of `plan` over d_data[batches][n] in place.
 // flags: STNTT_SIGNED_INPUT, STNTT_BITREV_IN, STNTT_BITREV_OUT.  mul_in, when
 // set, is multiplied into the input word by word as the first pass loads it
-// (with bit-reversed input only on multi-pass schedules).
+// (with bit-reversed input only on multi-pass schedules).  Gathered form:
+// in_tab / out_tab (device arrays of `batches` pointers) replace d_data as
+// the source of the first pass and the destination of the last, and
+// ext_scratch (2 n batches words) replaces the context's scratch, so that
+// nothing is allocated (graph capture).
 template <class FP>
 static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_data, uint64_t batches,
-                         uint32_t flags, hipStream_t s, const typename FP::T *mul_in = nullptr) {
+                         uint32_t flags, hipStream_t s, const typename FP::T *mul_in = nullptr,
+                         const typename FP::T *const *in_tab = nullptr, typename FP::T *const *out_tab = nullptr,
+                         typename FP::T *ext_scratch = nullptr) {
     using T = typename FP::T;
     const uint32_t L = plan.key.log_n;
     if (batches == 0) return STNTT_OK;
@@ -333,9 +355,13 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
         const int halves = sch.npasses > 2 ? 2 : 1;
-        int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
-        if (rc) return rc;
-        scratch[0] = (T *)ctx->scratch;
+        if (ext_scratch) {
+            scratch[0] = ext_scratch;
+        } else {
+            int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
+            if (rc) return rc;
+            scratch[0] = (T *)ctx->scratch;
+        }
         scratch[1] = halves > 1 ? scratch[0] + total : nullptr;
     }
     PassArgs<FP> a{};
@@ -365,6 +391,8 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         a.flags = (first ? (PASS_FIRST | (flags & STNTT_SIGNED_INPUT ? PASS_SIGNED : 0)) : 0) |
                   (last ? (PASS_LAST | (inv ? PASS_SCALE : 0)) : 0);
         a.mul_in = first ? mul_in : nullptr;
+        a.in_tab = first ? in_tab : nullptr;
+        a.out_tab = last ? out_tab : nullptr;
         bool perm = false;
         if (sch.single) {
             a.flags |= (rev_in ? PASS_BITREV_IN : 0) | (rev_out ? PASS_BITREV_OUT : 0);
@@ -503,29 +531,57 @@ static int batch_validate_split(uint32_t L, bool global_path, uint32_t nsub, uin
     return STNTT_OK;
 }
 
+static uint64_t elem_bytes(uint32_t field) { return field == STNTT_FIELD_U32 ? 4 : 8; }
+
+// largest transform a batch descriptor may hold
+constexpr uint32_t BATCH_MAX_LOG = 24;
+
 static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch *b) {
-    std::vector<BatchTask> tasks(count);
-    uint64_t max_stream = 0;  // largest streamed transform, in 64-bit words
+    std::vector<BatchTask> tasks;
+    tasks.reserve(count);
+    std::map<std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint64_t>, size_t> gidx;
+    std::vector<std::vector<void *>> gsrc, gdst;
     for (size_t i = 0; i < count; ++i) {
         const stntt_desc &d = descs[i];
-        BatchTask &t = tasks[i];
         if (d.field != STNTT_FIELD_U32 && d.field != STNTT_FIELD_GOLDILOCKS)
             return fail(STNTT_EINVAL, "descriptor %zu: bad field %u", i, d.field);
         if (d.direction > 1) return fail(STNTT_EINVAL, "descriptor %zu: bad direction %u", i, d.direction);
-        if (d.log2n > 18) return fail(STNTT_EINVAL, "descriptor %zu: log2n %u above the batched limit 18", i, d.log2n);
+        if (d.log2n > BATCH_MAX_LOG)
+            return fail(STNTT_EINVAL, "descriptor %zu: log2n %u above the batched limit %u", i, d.log2n, BATCH_MAX_LOG);
         if (d.flags & ~STNTT_SIGNED_INPUT)
             return fail(STNTT_EINVAL, "descriptor %zu: flags 0x%x (batches take natural order only)", i, d.flags);
         if (!d.src || !d.dst) return fail(STNTT_EINVAL, "descriptor %zu: null buffer", i);
         const Plan *plan;
         int rc;
-        if (d.field == STNTT_FIELD_U32) {
-            if ((rc = check_u32_params(d.log2n, d.root, d.mod, d.direction))) return rc;
-            if ((rc = get_plan(ctx, STNTT_FIELD_U32, d.log2n, d.root % d.mod, d.mod, d.direction, &plan))) return rc;
-            t.rt32 = plan->rt_u32;
-        } else {
-            if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, d.log2n, 0, GL_P, d.direction, &plan))) return rc;
-            t.rtgl = plan->rt_gl;
+        const uint64_t root = d.field == STNTT_FIELD_U32 ? d.root % d.mod : 0;
+        const uint64_t mod = d.field == STNTT_FIELD_U32 ? d.mod : GL_P;
+        if (d.field == STNTT_FIELD_U32 && (rc = check_u32_params(d.log2n, d.root, d.mod, d.direction))) return rc;
+        if ((rc = get_plan(ctx, (int)d.field, d.log2n, root, mod, (int)d.direction, &plan))) return rc;
+        const uint32_t sflag = (d.field == STNTT_FIELD_U32 && (d.flags & STNTT_SIGNED_INPUT)) ? STNTT_SIGNED_INPUT : 0;
+        if (batch_global_path(d.field, d.log2n)) {  // a group of the multi-pass path
+            auto key = std::make_tuple((int)d.field, (int)d.direction, d.log2n, sflag, root, mod);
+            auto it = gidx.find(key);
+            if (it == gidx.end()) {
+                BatchGroup g;
+                g.field = (int)d.field;
+                g.dir = (int)d.direction;
+                g.log2n = d.log2n;
+                g.flags = sflag;
+                g.root = root;
+                g.mod = mod;
+                g.plan = plan;
+                it = gidx.emplace(key, b->groups.size()).first;
+                b->groups.push_back(g);
+                gsrc.emplace_back();
+                gdst.emplace_back();
+            }
+            gsrc[it->second].push_back(const_cast<void *>(d.src));
+            gdst[it->second].push_back(d.dst);
+            continue;
         }
+        BatchTask t{};
+        if (d.field == STNTT_FIELD_U32) t.rt32 = plan->rt_u32;
+        else t.rtgl = plan->rt_gl;
         t.src = d.src;
         t.dst = d.dst;
         t.hi = plan->hi();
@@ -534,18 +590,16 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
         t.dir = d.direction;
         t.log_n = d.log2n;
         t.log_rmax = plan->log_rmax;
-        t.flags = PASS_FIRST | PASS_LAST | (d.direction == STNTT_INVERSE ? PASS_SCALE : 0) |
-                  ((d.field == STNTT_FIELD_U32 && (d.flags & STNTT_SIGNED_INPUT)) ? PASS_SIGNED : 0);
-        const bool gp = batch_global_path(d.field, d.log2n);
+        t.flags = PASS_FIRST | PASS_LAST | (d.direction == STNTT_INVERSE ? PASS_SCALE : 0) | (sflag ? PASS_SIGNED : 0);
         auto it = ctx->batch_split.find({(int)d.field, d.log2n});
         if (it != ctx->batch_split.end()) {
             t.nsub = it->second.first;
             t.radix_logs = it->second.second;
         } else {
-            default_batch_split(d.log2n, gp, &t.nsub, &t.radix_logs);
+            default_batch_split(d.log2n, false, &t.nsub, &t.radix_logs);
         }
-        if (d.log2n > 0 && (rc = batch_validate_split(d.log2n, gp, t.nsub, t.radix_logs))) return rc;
-        if (gp) max_stream = std::max<uint64_t>(max_stream, 1ull << d.log2n);
+        if (d.log2n > 0 && (rc = batch_validate_split(d.log2n, false, t.nsub, t.radix_logs))) return rc;
+        tasks.push_back(t);
     }
     // largest first (longest-processing-time order) so the tail is short
     std::stable_sort(tasks.begin(), tasks.end(), [](const BatchTask &x, const BatchTask &y) {
@@ -556,13 +610,25 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_batch_kernel, BATCH_THREADS, 0));
     if (per_cu < 1) per_cu = 1;
-    b->grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, std::max<size_t>(count, 1));
-    b->count = (uint32_t)count;
-    b->scratch_stride = max_stream;
-    HIP_TRY(hipMalloc(&b->d_tasks, std::max<size_t>(count, 1) * sizeof(BatchTask)));
-    if (count) HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), count * sizeof(BatchTask), hipMemcpyHostToDevice));
+    b->count = (uint32_t)tasks.size();
+    b->grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, std::max<size_t>(tasks.size(), 1));
+    b->scratch_stride = 0;  // no queue task streams through global memory
+    HIP_TRY(hipMalloc(&b->d_tasks, std::max<size_t>(tasks.size(), 1) * sizeof(BatchTask)));
+    if (!tasks.empty())
+        HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&b->d_counter, sizeof(uint32_t)));
-    if (max_stream) HIP_TRY(hipMalloc(&b->d_scratch, (size_t)b->grid * max_stream * sizeof(uint64_t)));
+    size_t scratch = 0;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        BatchGroup &g = b->groups[gi];
+        g.count = (uint32_t)gsrc[gi].size();
+        const size_t tab = g.count * sizeof(void *);
+        HIP_TRY(hipMalloc((void **)&g.d_src, tab));
+        HIP_TRY(hipMalloc((void **)&g.d_dst, tab));
+        HIP_TRY(hipMemcpy(g.d_src, gsrc[gi].data(), tab, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.d_dst, gdst[gi].data(), tab, hipMemcpyHostToDevice));
+        scratch = std::max(scratch, (size_t)2 * g.count * (elem_bytes(g.field) << g.log2n));
+    }
+    if (scratch) HIP_TRY(hipMalloc(&b->d_group_scratch, scratch));
     b->descs.assign(descs, descs + count);
     return STNTT_OK;
 }
@@ -570,10 +636,23 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
 static int batch_launch(stntt_batch *b, hipStream_t s) {
     hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, b->d_counter);
     HIP_TRY(hipGetLastError());
-    if (b->count == 0) return STNTT_OK;
-    hipLaunchKernelGGL(ntt_batch_kernel, dim3(b->grid), dim3(BATCH_THREADS), 0, s, b->d_tasks, b->count,
-                       b->d_counter, b->d_scratch, b->scratch_stride);
-    HIP_TRY(hipGetLastError());
+    if (b->count) {
+        hipLaunchKernelGGL(ntt_batch_kernel, dim3(b->grid), dim3(BATCH_THREADS), 0, s, b->d_tasks, b->count,
+                           b->d_counter, b->d_scratch, b->scratch_stride);
+        HIP_TRY(hipGetLastError());
+    }
+    for (const BatchGroup &g : b->groups) {
+        int rc;
+        if (g.field == STNTT_FIELD_GOLDILOCKS)
+            rc = run_transform<FieldGL>(b->ctx, *g.plan, nullptr, g.count, g.flags, s, nullptr,
+                                        (const uint64_t *const *)g.d_src, (uint64_t *const *)g.d_dst,
+                                        (uint64_t *)b->d_group_scratch);
+        else
+            rc = run_transform<FieldU32>(b->ctx, *g.plan, nullptr, g.count, g.flags, s, nullptr,
+                                         (const uint32_t *const *)g.d_src, (uint32_t *const *)g.d_dst,
+                                         (uint32_t *)b->d_group_scratch);
+        if (rc) return rc;
+    }
     return STNTT_OK;
 }
 
@@ -583,12 +662,16 @@ static void batch_free(stntt_batch *b) {
     if (b->d_tasks) (void)hipFree(b->d_tasks);
     if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->d_scratch) (void)hipFree(b->d_scratch);
+    for (BatchGroup &g : b->groups) {
+        if (g.d_src) (void)hipFree(g.d_src);
+        if (g.d_dst) (void)hipFree(g.d_dst);
+    }
+    if (b->d_group_scratch) (void)hipFree(b->d_group_scratch);
 }
 
 // ---------------------------------------------------------------------------
 // serving
 // ---------------------------------------------------------------------------
-static uint64_t elem_bytes(uint32_t field) { return field == STNTT_FIELD_GOLDILOCKS ? 8 : 4; }
 
 static int server_check_request(const stntt_request &r) {
     if (r.field != STNTT_FIELD_U32 && r.field != STNTT_FIELD_GOLDILOCKS) return fail(STNTT_EINVAL, "bad field");

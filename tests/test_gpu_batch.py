@@ -152,8 +152,64 @@ def test_out_of_place_descriptors(ctx):
 def test_batch_rejects_bad_descriptors(ctx):
     buf = ctx.alloc(8 << 19)
     with pytest.raises(ValueError):
-        S.Batch(ctx, [("goldilocks", False, 19, buf, 0, P)])   # above the batched limit
+        S.Batch(ctx, [("goldilocks", False, 25, buf, 0, P)])   # above the batched limit
     with pytest.raises(ValueError):
         S.Batch(ctx, [("u32", False, 4, buf, 2, 17)])          # 2 is not of order 16 mod 17
     with pytest.raises(ValueError):
         S.set_batch_split(ctx, "goldilocks", 16, [4, 4, 4, 3, 1])  # odd count on a streamed size
+
+
+def test_grouped_single_pass_with_tables(ctx):
+    """130 Goldilocks 2^13 transforms form one group whose schedule is the
+    single-pass kernel over pointer tables; half of them out of place."""
+    L, cnt = 13, 130
+    rng = np.random.default_rng(17)
+    xs = [rng.integers(0, 2**64, size=1 << L, dtype=np.uint64) for _ in range(cnt)]
+    srcs = [ctx.upload(x) for x in xs]
+    dsts = [s if i % 2 == 0 else ctx.alloc(x.nbytes) for i, (s, x) in enumerate(zip(srcs, xs))]
+    arr = (_lib.Desc * cnt)()
+    for i in range(cnt):
+        arr[i] = _lib.Desc(1, i % 3 == 0, L, 0, 0, 0, srcs[i].ptr, dsts[i].ptr)
+    h = ctypes.c_void_p()
+    _lib.check(_lib.lib().stntt_batch_create(ctx.handle, arr, cnt, ctypes.byref(h)))
+    _lib.check(_lib.lib().stntt_batch_run(h, None))
+    ctx.sync()
+    for i, x in enumerate(xs):
+        want = O.inverse_gl(x, L) if i % 3 == 0 else O.stockham_gl(x, L)
+        assert np.array_equal(dsts[i].copy_to(np.empty_like(x)), want), i
+        if i % 2:
+            assert np.array_equal(srcs[i].copy_to(np.empty_like(x)), x), i  # source untouched
+    _lib.lib().stntt_batch_destroy(h)
+
+
+def test_large_descriptors_in_a_batch(ctx):
+    """Sizes past the old batched limit, signed 32-bit inputs, both
+    directions, graph capture and replay."""
+    rng = np.random.default_rng(23)
+    specs = [("goldilocks", False, 22), ("u32", True, 21), ("goldilocks", True, 20), ("u32", False, 19),
+             ("goldilocks", False, 22)]
+    items, descs = [], []
+    for field, inv, L in specs:
+        n = 1 << L
+        if field == "goldilocks":
+            x, root, mod = rng.integers(0, P, size=n, dtype=np.uint64), 0, P
+        else:
+            root, mod = S.find_params(n, 11)
+            x = rng.integers(0, mod, size=n, dtype=np.uint64).astype(np.uint32)
+        buf = ctx.upload(x)
+        items.append((field, inv, L, x, root, mod, buf))
+        descs.append((field, inv, L, buf, root, mod))
+    b = S.Batch(ctx, descs)
+    b.capture()
+    for rep in (1, 2):
+        b.run_graph()
+        ctx.sync()
+        for field, inv, L, x, root, mod, buf in items:
+            y = x
+            for _ in range(rep):
+                if field == "goldilocks":
+                    y = O.inverse_gl(y, L) if inv else O.stockham_gl(y, L)
+                else:
+                    y = O.inverse_u32(y, 1 << L, root, mod) if inv else O.stockham_u32(y, 1 << L, root, mod)
+            assert np.array_equal(buf.copy_to(np.empty_like(x)), y), (field, L, rep)
+    b.close()
