@@ -69,6 +69,20 @@ constexpr int tile_threads() {
 // contiguous chunk per column, written back through LDS.
 enum : int { PERM_NONE = 0, PERM_IN = 1, PERM_OUT = 2 };
 
+// Column halves.  A radix-512 tile is 64 KiB of LDS, which holds a CU to two
+// workgroups.  For odd LOGR, r1 = 2 r2 and sub-step 2 has two groups per
+// thread anyway, so the exchange can run in two rounds of C/2 columns
+// through a 32 KiB tile: sub-step 1 keeps its outputs in registers, the
+// threads of one column half write them, every thread runs one sub-step 2
+// group of that half, then the other half follows.  Two more barriers, and
+// twice the resident workgroups.  Not for the permuted variants, which stage
+// the whole tile.
+template <class FP, int LOGR, int PERM>
+constexpr bool tile_half() {
+    constexpr int R = 1 << LOGR, C = 1 << tile_logc<FP>();
+    return PERM == PERM_NONE && (size_t)R * C * sizeof(typename FP::T) > 32768 && (LOGR & 1);
+}
+
 template <class FP, int LOGR, int DIR, bool J1, int NT, int PERM = PERM_NONE>
 __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block, typename FP::T *lds,
                                           typename FP::Tw *twl) {
@@ -85,6 +99,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     constexpr int SP = (R * C + NT - 1) / NT;  // staged words per thread (PERM_IN / PERM_OUT)
     static_assert(PERM != PERM_IN || (J1 && G1 == 1), "bit-reversed input: first pass, one group per thread");
     static_assert(PERM != PERM_OUT || !J1, "bit-reversed output: last pass only");
+    constexpr bool HALF = tile_half<FP, LOGR, PERM>();
+    static_assert(!HALF || (NT == tile_threads<FP, LOGR>() && G1 == 1 && NG2 == 2 * NT), "column halves layout");
+    constexpr int CL = HALF ? C / 2 : C;  // columns of the LDS image
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
@@ -137,6 +154,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         }
         __syncthreads();
     }
+    [[maybe_unused]] T xo[HALF ? r1 : 1];  // HALF: sub-step 1 outputs until the thread's round
+    [[maybe_unused]] uint32_t xo_c = 0, xo_g = 0;
 #pragma unroll
     for (int gi = 0; gi < G1; ++gi) {
         const uint32_t q = tid + gi * NT;
@@ -192,30 +211,26 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             for (int k = 0; k < r1; ++k) {
                 T x = v[bitrev_c(k, LR1)];
                 if (k != 0) x = FP::mul(x, w1[k], a.rt);
-                const uint32_t row = r1 * g + k;
-                lds[row * C + (c ^ (row & (C - 1)))] = x;
+                if constexpr (HALF) {
+                    xo[k] = x;
+                } else {
+                    const uint32_t row = r1 * g + k;
+                    lds[row * C + (c ^ (row & (C - 1)))] = x;
+                }
             }
         }
+        xo_c = c;
+        xo_g = g;
     }
-    __syncthreads();
+    if constexpr (!HALF) __syncthreads();
 
     // ---- sub-step 2: LDS -> registers -> DFT_r2 -> pass twiddle -> global --
     const auto roots2 = FP::template load_roots<LR2, DIR>(a.hi, a.log_rmax);
     // PERM_OUT: outputs are kept in registers until the LDS tile is free
     [[maybe_unused]] T xs[PERM == PERM_OUT ? G2 : 1][PERM == PERM_OUT ? r2 : 1];
     [[maybe_unused]] uint32_t xc[G2], xg2[G2];
-#pragma unroll
-    for (int gi = 0; gi < G2; ++gi) {
-        const uint32_t q = tid + gi * NT;
-        if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) continue;
-        uint32_t c, g2;
-        if constexpr (J1) {  // outputs of a column are contiguous: lanes walk g2
-            g2 = q & (r1 - 1);
-            c = q >> LR1;
-        } else {  // outputs of a row are contiguous: lanes walk c
-            c = q & (C - 1);
-            g2 = q >> LOGC;
-        }
+    // one sub-step 2 group: column c (lc within the LDS image), group g2
+    auto substep2 = [&](uint32_t c, uint32_t g2, uint32_t lc, int gi) {
         // J = 1: output k = k2 r1 + g2 of column s = t0 + c takes w^(s k).
         // Its factors are loaded before the LDS reads so their latency
         // overlaps them and the DFT.  Goldilocks walks a running product
@@ -245,7 +260,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 #pragma unroll
         for (int m = 0; m < r2; ++m) {
             const uint32_t row = g2 + m * r1;
-            v[m] = lds[row * C + (c ^ (row & (C - 1)))];
+            v[m] = lds[row * CL + (lc ^ (row & (CL - 1)))];
         }
         dft_regs<FP, LR2, DIR>(v, roots2, a.rt);
         [[maybe_unused]] Tw cur, step;
@@ -281,6 +296,40 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         }
         xc[gi] = c;
         xg2[gi] = g2;
+    };
+    if constexpr (HALF) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if ((xo_c >> (LOGC - 1)) == (uint32_t)h) {  // this thread's column is in half h
+#pragma unroll
+                for (int k = 0; k < r1; ++k) {
+                    const uint32_t row = r1 * xo_g + k, lc = xo_c & (CL - 1);
+                    lds[row * CL + (lc ^ (row & (CL - 1)))] = xo[k];
+                }
+            }
+            __syncthreads();
+            uint32_t lc, g2;
+            if constexpr (J1) {  // lanes walk g2
+                g2 = tid & (r1 - 1);
+                lc = tid >> LR1;
+            } else {  // lanes walk c
+                lc = tid & (CL - 1);
+                g2 = tid >> (LOGC - 1);
+            }
+            substep2(h * CL + lc, g2, lc, h);
+            if (h == 0) __syncthreads();  // round 0's reads before round 1's writes
+        }
+    } else {
+#pragma unroll
+        for (int gi = 0; gi < G2; ++gi) {
+            const uint32_t q = tid + gi * NT;
+            if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) continue;
+            if constexpr (J1) {  // outputs of a column are contiguous: lanes walk g2
+                substep2(q >> LR1, q & (r1 - 1), q >> LR1, gi);
+            } else {  // outputs of a row are contiguous: lanes walk c
+                substep2(q & (C - 1), q >> LOGC, q & (C - 1), gi);
+            }
+        }
     }
     if constexpr (PERM == PERM_OUT) {
         __syncthreads();  // every sub-step 2 read of the tile is done
@@ -310,7 +359,7 @@ template <class FP, int LOGR, int DIR, bool J1, int PERM = PERM_NONE>
 __global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(PassArgs<FP> a) {
     constexpr int C = 1 << tile_logc<FP>();
     constexpr int R = 1 << LOGR;
-    __shared__ typename FP::T lds[R * C];
+    __shared__ typename FP::T lds[R * C / (tile_half<FP, LOGR, PERM>() ? 2 : 1)];
     __shared__ typename FP::Tw twl[J1 ? 1 : R];
     tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>(), PERM>(a, blockIdx.x, lds, twl);
 }
