@@ -53,6 +53,10 @@ def parse():
                     help="also run a plain device copy of the same buffer (counter calibration)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--legs", default="n20,orders,field64,batched,serving",
+                    help="comma-separated extra legs to run after the headline")
+    ap.add_argument("--no-batch-plain", action="store_true",
+                    help="batched leg without the per-transform plain comparison")
     ap.add_argument("--serve-count", type=int, default=400)
     return ap.parse_args()
 
@@ -217,9 +221,11 @@ def bench_field64(S, ctx, steps: int) -> dict:
     return out
 
 
-def bench_batched(S, ctx, count: int, reps: int) -> dict:
-    """4096 transforms, log2 n ~ U{10..18}, field and direction 50/50: one
-    persistent launch (graph replay) versus per-transform plain launches."""
+def bench_batched(S, ctx, count: int, reps: int, plain: bool = True) -> dict:
+    """4096 transforms, log2 n ~ U{10..18}, field and direction 50/50: the
+    batch (one persistent launch for the LDS-resident sizes plus grouped
+    tile passes for the rest, replayed as one HIP graph) versus
+    per-transform plain launches."""
     from stntt_amd.workloads import mixed_batch
     specs = mixed_batch(seed=42, count=count, lo=10, hi=18)
     rng = np.random.default_rng(4)
@@ -250,22 +256,26 @@ def bench_batched(S, ctx, count: int, reps: int) -> dict:
         batch.run_graph()
     ctx.sync()
     t_batch = (time.perf_counter() - t0) / reps
-    batch.run_plain()
-    ctx.sync()
-    t0 = time.perf_counter()
-    plain_reps = max(1, reps // 4)
-    for _ in range(plain_reps):
+    t_plain = float("nan")
+    if plain:
         batch.run_plain()
-    ctx.sync()
-    t_plain = (time.perf_counter() - t0) / plain_reps
+        ctx.sync()
+        t0 = time.perf_counter()
+        plain_reps = max(1, reps // 4)
+        for _ in range(plain_reps):
+            batch.run_plain()
+        ctx.sync()
+        t_plain = (time.perf_counter() - t0) / plain_reps
     batch.close()
     for b in bufs:
         b.free()
-    return {"transforms": count, "elements": elems, "algorithmic_bytes": nbytes,
-            "batched_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
-            "batched_transforms_s": round(count / t_batch, 1), "plain_transforms_s": round(count / t_plain, 1),
-            "speedup": round(t_plain / t_batch, 2),
-            "batched_hbm_frac": round(nbytes / t_batch / 1e9 / HBM_PEAK_GBS, 4)}
+    out = {"transforms": count, "elements": elems, "algorithmic_bytes": nbytes,
+           "batched_ms": round(t_batch * 1e3, 3), "batched_transforms_s": round(count / t_batch, 1),
+           "batched_hbm_frac": round(nbytes / t_batch / 1e9 / HBM_PEAK_GBS, 4)}
+    if plain:
+        out.update({"plain_ms": round(t_plain * 1e3, 3), "plain_transforms_s": round(count / t_plain, 1),
+                    "speedup": round(t_plain / t_batch, 2)})
+    return out
 
 
 def bench_serving(S, count: int, workers: int) -> dict:
@@ -422,15 +432,20 @@ def main() -> int:
         "hbm_frac_transform": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "pass_ms": [round(v, 5) for v in mean_pass],
     }
-    if not args.no_extra:
+    legs = set() if args.no_extra else set(args.legs.split(","))
+    if "n20" in legs:
         log("[bench] n=2^20 forward+inverse, both fields")
         out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
+    if "orders" in legs:
         log("[bench] element orders and polynomial multiply")
         out["orders_polymul"] = bench_orders_polymul(S, ctx, 20)
+    if "field64" in legs:
         log("[bench] 64-bit prime field")
         out["field64_2^24"] = bench_field64(S, ctx, 20)
+    if "batched" in legs:
         log(f"[bench] batched mixed workload ({args.batch_count} transforms)")
-        out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10)
+        out["batched_mixed"] = bench_batched(S, ctx, args.batch_count, 10, plain=not args.no_batch_plain)
+    if "serving" in legs:
         log("[bench] serving stream")
         out["serving"] = bench_serving(S, args.serve_count, 4)
     if not args.no_cpu:

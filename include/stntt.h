@@ -201,7 +201,9 @@ int stntt_batch_destroy(stntt_batch *b);
 int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs);
 /* Tuning: stages per tile pass of the multi-pass path for (field, log2n),
    one log2 radix per pass (first pass first), as chosen by the autotuner;
-   npasses = 0 restores the built-in split.  Each pass covers 4..9 stages. */
+   npasses = 0 restores the built-in split.  Each pass covers 4..9 stages.
+   Applies to single transforms (batches == 1, what the autotuner times);
+   batches keep the built-in fewest-pass split. */
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr);
 
 /* ---- serving: continuous batching under an HBM budget -------------------

@@ -57,7 +57,7 @@ def pass_candidates(L: int, logc: int, lo: int = 4, hi: int = 9) -> list[list[in
             out.append(list(parts))
     # fewest passes first, then the most even splits
     out.sort(key=lambda p: (len(p), max(p) - min(p)))
-    return out[:8]
+    return out[:12]
 
 
 def batch_candidates(L: int, streamed: bool) -> list[list[int]]:

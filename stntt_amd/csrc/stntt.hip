@@ -338,7 +338,9 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     if (batches == 0) return STNTT_OK;
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
     Schedule sch = make_schedule<FP>(L, batches);
-    if (!sch.single) {
+    // tuned splits were measured on one transform (latency); a batch is
+    // throughput-bound and keeps the fewest-pass default
+    if (!sch.single && batches == 1) {
         auto it = ctx->pass_split.find({plan.key.field, L});
         if (it != ctx->pass_split.end()) {
             sch.npasses = (int)it->second.size();
