@@ -19,6 +19,8 @@ constexpr int PASS_THREADS = 256;
 constexpr int MAX_LOGR = 14;        // largest pass radix; also the size of the hi twiddle factor
 constexpr int MIN_LOGR_TILE = 4;    // smallest radix of a tile pass
 constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile)
+constexpr int NARROW_MAX_LOGR = 9;  // single-pass sizes with a one-transform, one-wave variant
+constexpr int NARROW_THREADS = 64;
 
 // log2 of the tile size in elements: 64 KiB of LDS per workgroup
 template <class FP>

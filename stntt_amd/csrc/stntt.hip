@@ -269,8 +269,15 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
 // kernel dispatch: one template instance per (field, dir, LOGR, single)
 // ---------------------------------------------------------------------------
 template <class FP, int DIR, int LOGR>
-static void launch_single(const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
+static void launch_single(const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s) {
     constexpr int LOGC = pass_logc<FP>(LOGR, true);
+    if constexpr (LOGR <= NARROW_MAX_LOGR) {
+        if (narrow) {  // one transform, one wave
+            hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, 0, NARROW_THREADS, DIR, true>), dim3(grid),
+                               dim3(NARROW_THREADS), 0, s, a);
+            return;
+        }
+    }
     hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, LOGC, PASS_THREADS, DIR, true>), dim3(grid), dim3(PASS_THREADS), 0,
                        s, a);
 }
@@ -289,9 +296,9 @@ static void launch_tile(bool first, bool perm, const PassArgs<FP> &a, uint32_t g
 }
 
 template <class FP, int DIR, int... L>
-static void launch_single_impl(int logr, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
+static void launch_single_impl(int logr, const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s,
                                std::integer_sequence<int, L...>) {
-    ((logr == L + 1 ? (launch_single<FP, DIR, L + 1>(a, grid, s), 0) : 0), ...);
+    ((logr == L + 1 ? (launch_single<FP, DIR, L + 1>(a, grid, narrow, s), 0) : 0), ...);
 }
 
 template <class FP, int DIR, int... L>
@@ -301,15 +308,15 @@ static void launch_tile_impl(int logr, bool first, bool perm, const PassArgs<FP>
 }
 
 template <class FP>
-static void launch_pass(int dir, bool single, bool first, bool perm, int logr, const PassArgs<FP> &a, uint32_t grid,
-                        hipStream_t s) {
+static void launch_pass(int dir, bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
+                        uint32_t grid, hipStream_t s) {
     using SeqS = std::make_integer_sequence<int, MAX_LOGR_SINGLE<FP>>;
     using SeqT = std::make_integer_sequence<int, MAX_LOGR_TILE - MIN_LOGR_TILE + 1>;
     if (dir == 0) {
-        if (single) launch_single_impl<FP, 0>(logr, a, grid, s, SeqS{});
+        if (single) launch_single_impl<FP, 0>(logr, a, grid, narrow, s, SeqS{});
         else launch_tile_impl<FP, 0>(logr, first, perm, a, grid, s, SeqT{});
     } else {
-        if (single) launch_single_impl<FP, 1>(logr, a, grid, s, SeqS{});
+        if (single) launch_single_impl<FP, 1>(logr, a, grid, narrow, s, SeqS{});
         else launch_tile_impl<FP, 1>(logr, first, perm, a, grid, s, SeqT{});
     }
 }
@@ -402,14 +409,21 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             perm = (first && rev_in) || (last && rev_out);
         }
         uint32_t grid;
+        bool narrow = false;
         if (sch.single) {
             uint64_t c = 1ull << pass_logc<FP>(logr, true);
             grid = (uint32_t)((batches + c - 1) / c);
+            // too few wide tiles to fill the chip: one transform per one-wave
+            // workgroup instead (a wide tile computes all C columns)
+            if (logr <= NARROW_MAX_LOGR && grid < 2 * (uint64_t)ctx->num_cus && c > 1) {
+                narrow = true;
+                grid = (uint32_t)batches;
+            }
         } else {
             a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> tile_logc<FP>());
             grid = (uint32_t)(batches * a.tiles_per_transform);
         }
-        launch_pass<FP>(plan.key.dir, sch.single, first, perm, logr, a, grid, s);
+        launch_pass<FP>(plan.key.dir, sch.single, first, perm, narrow, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
         src = dst;
