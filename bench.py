@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+VALU_PEAK_G = 1024 * 2.4 / 4  # wave64 VALU instructions per ns: 256 CUs x 4 SIMDs, 2.4 GHz, 4 cycles each
 METRIC = "Goldilocks 2^24 NTT GElem/s, %HBM peak; batched-vs-plain transforms/s"
 P = 0xFFFFFFFF00000001
 
@@ -402,14 +403,16 @@ def main() -> int:
     achieved = alg_bytes / (mean_pass[dom] * 1e-3) / 1e9
     transform_ms = sum(mean_pass)
     traffic = None
+    valu_instr = None
     if os.path.exists(args.traffic_json):
         try:
-            tj = json.load(open(args.traffic_json))
-            traffic = tj.get(f"goldilocks_2^{L}", {}).get("pass_bytes")
-            if isinstance(traffic, list):
-                traffic = traffic[dom] if dom < len(traffic) else None
-        except Exception:
-            traffic = None
+            prof = json.load(open(args.traffic_json)).get(f"goldilocks_2^{L}", {})
+            if len(prof.get("pass_bytes", [])) == npass:
+                traffic = prof["pass_bytes"][dom]
+            if len(prof.get("pass_valu_instr", [])) == npass:
+                valu_instr = prof["pass_valu_instr"][dom]
+        except (OSError, ValueError, KeyError, TypeError):
+            traffic = valu_instr = None
 
     out = {
         "metric": METRIC,
@@ -430,6 +433,14 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"pass {dom} of {npass}", "kernel_ms": round(mean_pass[dom], 5)},
         "hbm_frac_transform": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        # the bound that actually limits this kernel: VALU issue.  Wave64
+        # instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/) over
+        # the live mean duration, against 1024 SIMDs x 2.4 GHz / 4 cycles.
+        "valu_roofline": None if valu_instr is None else {
+            "bound": "valu", "achieved": round(valu_instr / (mean_pass[dom] * 1e-3) / 1e9, 1),
+            "peak": VALU_PEAK_G, "unit": "G wave-instr/s",
+            "frac": round(valu_instr / (mean_pass[dom] * 1e-3) / 1e9 / VALU_PEAK_G, 4),
+            "instr_per_launch": valu_instr},
         "pass_ms": [round(v, 5) for v in mean_pass],
     }
     legs = set() if args.no_extra else set(args.legs.split(","))
