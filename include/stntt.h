@@ -69,6 +69,9 @@ const char *stntt_last_error(void);
    scratch pool.  Replaces the reference's process-global twiddle symbols
    (src/cuda/ntt/implementations/common.cuh:8-9). */
 int stntt_ctx_create(int device, stntt_ctx **out);
+/* Waits for the context's stream, frees its plans and scratch.  Batches made
+   on the context must be destroyed first; device buffers from
+   stntt_dev_alloc are the caller's and are not freed here. */
 int stntt_ctx_destroy(stntt_ctx *ctx);
 stntt_stream stntt_ctx_stream(stntt_ctx *ctx);
 
@@ -238,6 +241,7 @@ typedef struct stntt_server stntt_server;
 int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out);
 int stntt_server_destroy(stntt_server *srv);
 int stntt_server_host_alloc(stntt_server *srv, size_t bytes, void **host_ptr);
+/* srv may be null (the memory outlives the server that allocated it) */
 int stntt_server_host_free(stntt_server *srv, void *host_ptr);
 /* queue a request; never blocks.  A request that cannot fit the budget even
    alone is STNTT_EINVAL. */

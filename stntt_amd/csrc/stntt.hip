@@ -343,6 +343,11 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     using T = typename FP::T;
     const uint32_t L = plan.key.log_n;
     if (batches == 0) return STNTT_OK;
+    // kernel arguments and grids are 32-bit: one launch covers at most 2^31
+    // transforms and 2^31 workgroups
+    if (batches >= (1ull << 31) || (batches << L) >> L != batches || (batches << L) >= (1ull << 40))
+        return fail(STNTT_EINVAL, "%llu transforms of 2^%u words is more than one call handles",
+                    (unsigned long long)batches, L);
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
     Schedule sch = make_schedule<FP>(L, batches);
     // tuned splits were measured on one transform (latency); a batch is
@@ -354,6 +359,12 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             for (int i = 0; i < sch.npasses; ++i) sch.logr[i] = it->second[i];
         }
     }
+    if (!sch.single)
+        for (int p = 0; p < sch.npasses; ++p) {
+            const uint64_t tiles = batches * ((((1ull << L) >> sch.logr[p])) >> tile_logc<FP>());
+            if (tiles >= (1ull << 31))
+                return fail(STNTT_EINVAL, "%llu tiles exceed one launch", (unsigned long long)tiles);
+        }
     const bool inv = plan.key.dir == STNTT_INVERSE;
     const bool rev_in = flags & STNTT_BITREV_IN, rev_out = flags & STNTT_BITREV_OUT;
     if (mul_in && !sch.single && !rev_in)
@@ -421,7 +432,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             }
         } else {
             a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> tile_logc<FP>());
-            grid = (uint32_t)(batches * a.tiles_per_transform);
+            grid = (uint32_t)(batches * a.tiles_per_transform);  // checked before the first launch
         }
         launch_pass<FP>(plan.key.dir, sch.single, first, perm, narrow, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
@@ -1526,7 +1537,7 @@ int stntt_server_host_alloc(stntt_server *srv, size_t bytes, void **host_ptr) {
 }
 
 int stntt_server_host_free(stntt_server *srv, void *host_ptr) {
-    if (!srv) return fail(STNTT_EINVAL, "null server");
+    (void)srv;  // pinned host memory does not depend on the server; it may be gone
     HIP_TRY(hipHostFree(host_ptr));
     return STNTT_OK;
 }

@@ -16,6 +16,7 @@ Errors keep the reference's kinds: a non-power-of-two size raises ValueError
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -52,12 +53,20 @@ class Context:
         check(lib().stntt_ctx_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        # buffers and batches made on this context: released before it is
+        # destroyed (a batch must not outlive its context)
+        self._owned: weakref.WeakSet = weakref.WeakSet()
         if tuned:
             from .autotune import apply_table
             apply_table(self)
 
+    def _own(self, obj) -> None:
+        self._owned.add(obj)
+
     def close(self) -> None:
         if self.handle:
+            for obj in list(self._owned):
+                obj._release()
             lib().stntt_ctx_destroy(self.handle)
             self.handle = None
 
@@ -192,6 +201,7 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         check(lib().stntt_dev_alloc(ctx.handle, nbytes, ctypes.byref(p)))
         self.ctx, self.ptr, self.nbytes = ctx, p.value, nbytes
+        ctx._own(self)
 
     def copy_from(self, arr: np.ndarray, stream: int | None = None) -> None:
         arr = np.ascontiguousarray(arr)
@@ -206,9 +216,12 @@ class DeviceBuffer:
         return arr
 
     def free(self) -> None:
-        if self.ptr:
+        """Release the allocation (a no-op once freed or once the context is closed)."""
+        if self.ptr and self.ctx.handle:
             lib().stntt_dev_free(self.ctx.handle, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    _release = free
 
     def __del__(self):
         try:
@@ -443,6 +456,7 @@ class Batch:
         check(lib().stntt_batch_create(ctx.handle, arr, len(descs), ctypes.byref(h)))
         self.handle = h
         self.count = len(descs)
+        ctx._own(self)
 
     def run(self, stream: int | None = None) -> None:
         check(lib().stntt_batch_run(self.handle, stream))
@@ -457,9 +471,12 @@ class Batch:
         check(lib().stntt_batch_run_plain(self.handle, stream))
 
     def close(self) -> None:
-        if self.handle:
+        """Destroy the batch (a no-op once closed or once the context is closed)."""
+        if self.handle and self.ctx.handle:
             lib().stntt_batch_destroy(self.handle)
-            self.handle = None
+        self.handle = None
+
+    _release = close
 
     def __del__(self):
         try:
@@ -501,6 +518,7 @@ class PinnedArray:
         self.ptr = p.value
         buf = (ctypes.c_char * nbytes).from_address(self.ptr)
         self.array = np.frombuffer(buf, dtype=dtype, count=int(count))
+        server._owned.add(self)
 
     def free(self) -> None:
         if self.ptr:
@@ -516,6 +534,7 @@ class Server:
         h = ctypes.c_void_p()
         check(lib().stntt_server_create(device, hbm_budget_bytes, workers, ctypes.byref(h)))
         self.handle = h
+        self._owned: weakref.WeakSet = weakref.WeakSet()  # pinned arrays, freed after the server
 
     def pinned(self, count: int, dtype) -> PinnedArray:
         return PinnedArray(self, count, dtype)
@@ -548,9 +567,12 @@ class Server:
         check(lib().stntt_server_reset_stats(self.handle))
 
     def close(self) -> None:
+        """Finish every request, stop the server, then free its pinned arrays."""
         if self.handle:
             lib().stntt_server_destroy(self.handle)
             self.handle = None
+            for arr in list(self._owned):
+                arr.free()
 
     def __del__(self):
         try:

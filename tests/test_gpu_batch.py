@@ -213,3 +213,18 @@ def test_large_descriptors_in_a_batch(ctx):
                     y = O.inverse_u32(y, 1 << L, root, mod) if inv else O.stockham_u32(y, 1 << L, root, mod)
             assert np.array_equal(buf.copy_to(np.empty_like(x)), y), (field, L, rep)
     b.close()
+
+
+def test_closing_the_context_releases_its_batches_and_buffers():
+    """A batch or buffer may outlive an explicit Context.close(): the context
+    releases them first, and their own close/free are then no-ops."""
+    c = S.Context(0)
+    x = np.arange(1 << 12, dtype=np.uint64)
+    buf = c.upload(x)
+    b = S.Batch(c, [("goldilocks", False, 12, buf, 0, P)])
+    b.run()
+    c.sync()
+    c.close()
+    assert buf.ptr is None and b.handle is None
+    b.close()
+    buf.free()

@@ -135,3 +135,13 @@ def test_server_errors():
         srv.wait(987654321)                           # unknown id
     data.free()
     srv.close()
+
+
+def test_server_close_frees_pinned_arrays():
+    srv = S.Server(0, hbm_budget_bytes=RESERVE + (1 << 30), workers=1)
+    data = srv.pinned(1 << 10, np.uint64)
+    data.array[:] = np.arange(1 << 10, dtype=np.uint64)
+    srv.wait(srv.submit("goldilocks", 10, data))
+    srv.close()
+    assert data.ptr is None
+    data.free()  # no-op
