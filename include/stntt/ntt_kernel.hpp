@@ -154,8 +154,8 @@ inline void pwcNtt(std::vector<int> &vec0, std::vector<int> &vec1, int root, int
 template <typename T>
 void elemWiseMul(std::span<T> s0, std::span<T> s1, size_t mod) {
     if (s1.size() < s0.size()) throw std::out_of_range("s1 is shorter than s0");
-    for (size_t i = 0; i < s0.size(); ++i)
-        s0[i] = (T)(((unsigned __int128)(uint64_t)s0[i] * (uint64_t)s1[i]) % mod);
+    __extension__ typedef unsigned __int128 wide_t;  // exact products of 64-bit words
+    for (size_t i = 0; i < s0.size(); ++i) s0[i] = (T)(((wide_t)(uint64_t)s0[i] * (uint64_t)s1[i]) % mod);
 }
 
 // pwc_convolution: the cyclic convolution of two positive vectors of equal
