@@ -8,7 +8,9 @@ Two choices are tuned and installed into a context through the C ABI:
 The winners are written to tuning/gfx950.json next to this file and applied
 to every new Context (Context(tuned=False) skips it).
 
-    python -m stntt_amd.autotune            # on an MI355X
+    python -m stntt_amd.autotune            # on an MI355X: everything
+    python -m stntt_amd.autotune --fields u32 --parts pass
+        # re-tune a subset; entries not re-tuned are kept from the table
 """
 from __future__ import annotations
 
@@ -92,14 +94,19 @@ def _time(fn, sync, reps: int) -> float:
     return (time.perf_counter() - t0) / reps
 
 
-def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True) -> dict:
+def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True, fields=FIELDS,
+         parts=("pass", "batch")) -> dict:
     import stntt_amd as S
     from .ntt import set_batch_split, set_pass_split
     ctx = S.Context(0, tuned=False)
     rng = np.random.default_rng(1)
     table = {"device": "gfx950", "pass_split": {f: {} for f in FIELDS}, "batch_split": {f: {} for f in FIELDS},
              "measured_us": {"pass": {f: {} for f in FIELDS}, "batch": {f: {} for f in FIELDS}}}
-    for field in FIELDS:
+    if "pass" not in parts:
+        logs_pass = ()
+    if "batch" not in parts:
+        logs_batch = ()
+    for field in fields:
         logc = 4 if field == "goldilocks" else 5
         for L in logs_pass:
             n = 1 << L
@@ -159,13 +166,38 @@ def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True
     return table
 
 
-def main() -> int:
-    out = sys.argv[1] if len(sys.argv) > 1 else TABLE
-    table = tune()
-    os.makedirs(os.path.dirname(out), exist_ok=True)
-    with open(out, "w") as f:
+def merge(old: dict, new: dict) -> dict:
+    """new's entries over old's, per (kind, field, log2 n)."""
+    out = json.loads(json.dumps(old))
+    out["device"] = new.get("device", out.get("device"))
+    for kind in ("pass_split", "batch_split"):
+        for field, entries in new.get(kind, {}).items():
+            out.setdefault(kind, {}).setdefault(field, {}).update(entries)
+    for kind, per_field in new.get("measured_us", {}).items():
+        for field, entries in per_field.items():
+            out.setdefault("measured_us", {}).setdefault(kind, {}).setdefault(field, {}).update(entries)
+    return out
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("out", nargs="?", default=TABLE)
+    ap.add_argument("--fields", default=",".join(FIELDS), help="comma list of u32, goldilocks")
+    ap.add_argument("--parts", default="pass,batch", help="comma list of pass, batch")
+    args = ap.parse_args(argv)
+    fields = tuple(f for f in args.fields.split(",") if f)
+    parts = tuple(p for p in args.parts.split(",") if p)
+    if any(f not in FIELDS for f in fields) or any(p not in ("pass", "batch") for p in parts):
+        ap.error("unknown field or part")
+    table = tune(fields=fields, parts=parts)
+    if os.path.exists(args.out):
+        with open(args.out) as f:
+            table = merge(json.load(f), table)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
-    print("wrote", out)
+    print("wrote", args.out)
     return 0
 
 

@@ -3,12 +3,16 @@
 // The reference keeps values in signed int and reduces with C '%'
 // (src/ntt/ntt_util.cpp:88-94, src/cuda/math.cuh:10-16); its products
 // w * (a - b) overflow int32 from p = 65537 on (src/ntt/ntt_cpu.cpp:74,
-// src/cuda/ntt/arithmetic.cuh:23).  Here p is any runtime prime below 2^31
-// (every findParams prime up to n = 2^27), values are canonical in [0, p),
-// and products are exact:
-//   * twiddles carry a Shoup companion wq = floor(w * 2^32 / p), so w * x
-//     costs one mul_hi and two mul_lo (Shoup, "NTL: A library for doing
-//     number theory", MulModPrecon);
+// src/cuda/ntt/arithmetic.cuh:23).  Here p is any runtime odd prime below
+// 2^31 (every findParams prime up to n = 2^27), values are canonical in
+// [0, p), and products are exact:
+//   * data words stay in normal form; twiddles are stored in Montgomery
+//     form wm = w 2^32 mod p, so x * w = REDC(x wm) (Montgomery, "Modular
+//     multiplication without trial division", Math. Comp. 44, 1985): one
+//     32 x 32 -> 64 product, one mul_lo, one 64-bit multiply-add and a
+//     conditional subtraction;
+//   * the product of two twiddles, REDC(am bm) = (a b) 2^32 mod p, is again
+//     a Montgomery twiddle, so twiddle runs are running products;
 //   * a + b and a - b use the min(s, s - p) trick on unsigned words.
 #pragma once
 
@@ -18,11 +22,6 @@
 #define U32_HD __host__ __device__ __forceinline__
 
 namespace stntt {
-
-struct U32Tw {
-    uint32_t w;   // omega^e mod p
-    uint32_t wq;  // floor(w * 2^32 / p)
-};
 
 U32_HD uint32_t u32_add(uint32_t a, uint32_t b, uint32_t p) {
     uint32_t s = a + b;  // < 2p < 2^32
@@ -36,13 +35,27 @@ U32_HD uint32_t u32_sub(uint32_t a, uint32_t b, uint32_t p) {
     return d < t ? d : t;
 }
 
-// x * w mod p with w's Shoup companion; exact for any 32-bit x.
-U32_HD uint32_t u32_mul(uint32_t x, U32Tw w, uint32_t p) {
-    uint32_t q = (uint32_t)(((uint64_t)x * w.wq) >> 32);
-    uint32_t r = x * w.w - q * p;  // in [0, 2p)
-    uint32_t t = r - p;
-    return r < t ? r : t;
+// x wm 2^-32 mod p in [0, p) for any 32-bit x and wm < p (p odd, < 2^31),
+// pinv = -p^-1 mod 2^32.  t = x wm < 2^63; m = t pinv mod 2^32 makes
+// t + m p divisible by 2^32, t + m p < 2^64, and the quotient is below 2p.
+U32_HD uint32_t u32_mont(uint32_t x, uint32_t wm, uint32_t p, uint32_t pinv) {
+    const uint64_t t = (uint64_t)x * wm;
+    const uint32_t m = (uint32_t)t * pinv;
+    const uint32_t u = (uint32_t)(((uint64_t)m * p + t) >> 32);
+    const uint32_t r = u - p;
+    return u < r ? u : r;
 }
+
+// -p^-1 mod 2^32 for odd p: Newton's iteration doubles the correct low bits
+// (x = p is correct to 3 bits)
+inline uint32_t u32_neg_inverse(uint32_t p) {
+    uint32_t x = p;
+    for (int i = 0; i < 4; ++i) x *= 2u - p * x;
+    return 0u - x;
+}
+
+// w 2^32 mod p: the Montgomery form of a twiddle (host side)
+inline uint32_t u32_to_mont(uint64_t w, uint32_t p) { return (uint32_t)(((w % p) << 32) % p); }
 
 // high 64 bits of a 64 x 64 product
 U32_HD uint64_t u64_mulhi(uint64_t a, uint64_t b) {
@@ -53,9 +66,9 @@ U32_HD uint64_t u64_mulhi(uint64_t a, uint64_t b) {
 #endif
 }
 
-// a * b mod p for a, b < p (neither a twiddle with a companion): Barrett
-// with mu = floor(2^64 / p).  x = ab < 2^62; q = floor(x mu / 2^64) is
-// floor(x / p) or one less, so x - q p lies in [0, 2p) and fits 32 bits.
+// a * b mod p for two data words a, b < p: Barrett with mu = floor(2^64 / p).
+// x = ab < 2^62; q = floor(x mu / 2^64) is floor(x / p) or one less, so
+// x - q p lies in [0, 2p) and fits 32 bits.
 U32_HD uint32_t u32_mul_any(uint32_t a, uint32_t b, uint32_t p, uint64_t mu) {
     uint64_t x = (uint64_t)a * b;
     uint64_t q = u64_mulhi(x, mu);

@@ -85,9 +85,6 @@ struct FieldGL {
     };
     // runtime roots are not needed: omega_r^j is a power of two
     struct Roots {};
-    // a twiddle times a twiddle is a twiddle: running products are cheap
-    static constexpr bool RUNNING_TWIDDLES = true;
-
     static __device__ __forceinline__ T add(T a, T b, const Rt &) { return gl_add(a, b); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &) { return gl_sub(a, b); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &) { return gl_mul(a, w); }
@@ -117,23 +114,24 @@ struct FieldGL {
 
 struct FieldU32 {
     using T = uint32_t;
-    using Tw = U32Tw;
+    using Tw = uint32_t;  // w 2^32 mod p (Montgomery form)
     struct Rt {
-        uint32_t p;    // modulus, < 2^31
-        uint32_t bar;  // floor(2^32 / p)
-        uint32_t c32;  // 2^32 mod p
-        U32Tw ninv;    // n^-1 with Shoup companion
-        uint64_t mu;   // floor(2^64 / p), for products of two data words
+        uint32_t p;     // modulus, odd, < 2^31
+        uint32_t pinv;  // -p^-1 mod 2^32
+        uint32_t bar;   // floor(2^32 / p)
+        uint32_t c32;   // 2^32 mod p: also the Montgomery form of 1
+        uint32_t ninv;  // n^-1 in Montgomery form
+        uint32_t pad;
+        uint64_t mu;    // floor(2^64 / p), for products of two data words
     };
     struct Roots {
-        U32Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
+        Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
     };
-    // a product of twiddles needs a fresh Shoup companion (a division)
-    static constexpr bool RUNNING_TWIDDLES = false;
 
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32_sub(a, b, rt.p); }
-    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return u32_mul(a, w, rt.p); }
+    // data word (normal form) times twiddle: normal form
+    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return u32_mont(a, w, rt.p, rt.pinv); }
     static __device__ __forceinline__ T mulg(T a, T b, const Rt &rt) { return u32_mul_any(a, b, rt.p, rt.mu); }
     // omega_r^j = hi[j << (log_rmax - LR)] (hi[e] = omega_Rmax^e)
     template <int LR, int DIR>
@@ -146,24 +144,19 @@ struct FieldU32 {
     template <int LR, int DIR, int J>
     static __device__ __forceinline__ T sub_root_mul(T x, T y, const Roots &r, const Rt &rt) {
         if constexpr (J == 0) return u32_sub(x, y, rt.p);
-        return u32_mul(u32_sub(x, y, rt.p), r.w[J], rt.p);
+        return u32_mont(u32_sub(x, y, rt.p), r.w[J], rt.p, rt.pinv);
     }
     static __device__ __forceinline__ T load_in(T x, const Rt &rt, uint32_t flags) {
         if (flags & PASS_FIRST) return u32_reduce_input(x, rt.p, rt.bar, rt.c32, (flags & PASS_SIGNED) != 0);
         return x;
     }
     static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
-        if (flags & PASS_SCALE) x = u32_mul(x, rt.ninv, rt.p);
+        if (flags & PASS_SCALE) x = u32_mont(x, rt.ninv, rt.p, rt.pinv);
         return x;
     }
-    // twiddle hi * lo with a fresh Shoup companion (one division per twiddle)
-    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) {
-        uint32_t w = u32_mul(hi.w, lo, rt.p);
-        return {w, (uint32_t)(((uint64_t)w << 32) / rt.p)};
-    }
-    static __device__ __forceinline__ Tw tw_one(const Rt &rt) {
-        return {1u % rt.p, (uint32_t)(((uint64_t)(1u % rt.p) << 32) / rt.p)};
-    }
+    // twiddle hi * lo: the Montgomery product of two Montgomery words
+    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) { return u32_mont(hi, lo, rt.p, rt.pinv); }
+    static __device__ __forceinline__ Tw tw_one(const Rt &rt) { return rt.c32; }
 };
 
 // Any odd prime below 2^64 (field_m64.hpp): data and twiddles in Montgomery
@@ -175,9 +168,6 @@ struct FieldM64 {
     struct Roots {
         Tw w[16];  // omega_r^j (Montgomery form), j < r/2 (r <= 32)
     };
-    // a product of twiddles is one Montgomery product
-    static constexpr bool RUNNING_TWIDDLES = true;
-
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return m64_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return m64_sub(a, b, rt.p); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return m64_mul(a, w, rt.p, rt.pinv); }

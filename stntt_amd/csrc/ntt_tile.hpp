@@ -231,30 +231,20 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     [[maybe_unused]] uint32_t xc[G2], xg2[G2];
     // one sub-step 2 group: column c (lc within the LDS image), group g2
     auto substep2 = [&](uint32_t c, uint32_t g2, uint32_t lc, int gi) {
-        // J = 1: output k = k2 r1 + g2 of column s = t0 + c takes w^(s k).
-        // Its factors are loaded before the LDS reads so their latency
-        // overlaps them and the DFT.  Goldilocks walks a running product
-        // w^(s g2) (w^(s r1))^k2 (exact, so the same words as a lookup, from
-        // four loads per thread); the 32-bit field multiplies by hi and lo
-        // (a running product would need a new Shoup companion per step).
-        constexpr int NTW = !J1 ? 1 : (FP::RUNNING_TWIDDLES ? 2 : r2);
-        Tw th[NTW], tl[NTW];
+        // J = 1: output k = k2 r1 + g2 of column s = t0 + c takes w^(s k),
+        // walked as the running product w^(s g2) (w^(s r1))^k2.  Twiddle
+        // products are exact in every field (Goldilocks words, Montgomery
+        // words for the others), so these are the words a lookup would give,
+        // from four loads per thread.  They are issued before the LDS reads so
+        // their latency overlaps them and the DFT.
+        [[maybe_unused]] Tw th[2], tl[2];
         if constexpr (J1) {
             const uint32_t s = t0 + c;
-            if constexpr (FP::RUNNING_TWIDDLES) {
-                const uint32_t e0 = s * g2, es = s * r1;  // < n
-                th[0] = at_off(a.hi, e0 >> log_b);
-                tl[0] = at_off(a.lo, e0 & bmask);
-                th[1] = at_off(a.hi, es >> log_b);
-                tl[1] = at_off(a.lo, es & bmask);
-            } else {
-                uint32_t e = s * g2;
-#pragma unroll
-                for (int k2 = 0; k2 < r2; ++k2, e += s * r1) {
-                    th[k2] = at_off(a.hi, e >> log_b);
-                    tl[k2] = at_off(a.lo, e & bmask);
-                }
-            }
+            const uint32_t e0 = s * g2, es = s * r1;  // < n
+            th[0] = at_off(a.hi, e0 >> log_b);
+            tl[0] = at_off(a.lo, e0 & bmask);
+            th[1] = at_off(a.hi, es >> log_b);
+            tl[1] = at_off(a.lo, es & bmask);
         }
         T v[r2];
 #pragma unroll
@@ -264,9 +254,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         }
         dft_regs<FP, LR2, DIR>(v, roots2, a.rt);
         [[maybe_unused]] Tw cur, step;
-        if constexpr (J1 && FP::RUNNING_TWIDDLES) {
-            cur = FP::mul(th[0], tl[0], a.rt);
-            step = FP::mul(th[1], tl[1], a.rt);
+        if constexpr (J1) {
+            cur = FP::twiddle(th[0], tl[0], a.rt);
+            step = FP::twiddle(th[1], tl[1], a.rt);
         }
         // stores: a uniform base per k2 plus one 32-bit lane offset
         const uint32_t lane_out = J1 ? (c * R + g2) : ((g2 << a.log_j) + c);
@@ -275,13 +265,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             const uint32_t k = k2 * r1 + g2;
             T x = v[bitrev_c(k2, LR2)];
             if constexpr (J1) {
-                if constexpr (FP::RUNNING_TWIDDLES) {
-                    x = FP::mul(x, cur, a.rt);
-                    if (k2 + 1 < r2) cur = FP::mul(cur, step, a.rt);
-                } else {
-                    x = FP::mul(x, th[k2], a.rt);
-                    if (log_b) x = FP::mul(x, tl[k2], a.rt);
-                }
+                x = FP::mul(x, cur, a.rt);
+                if (k2 + 1 < r2) cur = FP::twiddle(cur, step, a.rt);
             } else {
                 if (sJ != 0) x = FP::mul(x, twl[k], a.rt);  // uniform branch
             }

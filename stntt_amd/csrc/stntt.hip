@@ -74,8 +74,6 @@ static uint64_t invmod(uint64_t a, uint64_t m) {
     return (uint64_t)t;
 }
 
-static inline U32Tw shoup(uint32_t w, uint32_t p) { return {w, (uint32_t)(((uint64_t)w << 32) / p)}; }
-
 
 // ---------------------------------------------------------------------------
 // twiddle plans: resident factor tables
@@ -100,10 +98,10 @@ struct Plan {
     FieldU32::Rt rt_u32{};
     FieldM64::Rt rt_m64{};
     const void *hi() const { return d_tables; }
-    // every field's table entry is 8 bytes (a word, a Shoup pair, a Montgomery word)
-    const void *lo() const { return (const char *)d_tables + (sizeof(uint64_t) << log_rmax); }
+    const void *lo() const { return (const char *)d_tables + (entry_bytes(key.field) << log_rmax); }
+    // a 32-bit Montgomery word, a Goldilocks word, a 64-bit Montgomery word
+    static size_t entry_bytes(int field) { return field == STNTT_FIELD_U32 ? 4 : 8; }
 };
-static_assert(sizeof(U32Tw) == 8, "table entries are 8 bytes");
 
 // Batched transforms above the LDS-resident size, grouped by everything that
 // selects a plan and flags: each group runs the multi-pass tile kernels over
@@ -231,7 +229,7 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
     uint64_t w = field == STNTT_FIELD_GOLDILOCKS ? (log_n ? powmod(7, (GL_P - 1) >> log_n, GL_P) : 1) : root % p;
     if (dir == STNTT_INVERSE) w = invmod(w, p);
     const uint64_t wstep = powmod(w, nlo, p);  // w^(n / Rmax): the hi table's ratio
-    plan->bytes = (size_t)(rmax + nlo) * sizeof(uint64_t);
+    plan->bytes = (size_t)(rmax + nlo) * Plan::entry_bytes(field);
     HIP_TRY(hipMalloc(&plan->d_tables, plan->bytes));
     int rc;
     if (field == STNTT_FIELD_GOLDILOCKS) {
@@ -240,11 +238,12 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
     } else if (field == STNTT_FIELD_U32) {
         const uint32_t p32 = (uint32_t)p;
         plan->rt_u32.p = p32;
+        plan->rt_u32.pinv = u32_neg_inverse(p32);
         plan->rt_u32.bar = (uint32_t)((1ull << 32) / p32);
         plan->rt_u32.c32 = (uint32_t)((1ull << 32) % p32);
-        plan->rt_u32.ninv = shoup((uint32_t)invmod(n % p32, p32), p32);
+        plan->rt_u32.ninv = u32_to_mont(invmod(n % p32, p32), p32);
         plan->rt_u32.mu = ~0ull / p32;
-        rc = gen_tables<FieldU32>(ctx, plan.get(), shoup((uint32_t)w, p32), shoup((uint32_t)wstep, p32), rmax, nlo,
+        rc = gen_tables<FieldU32>(ctx, plan.get(), u32_to_mont(w, p32), u32_to_mont(wstep, p32), rmax, nlo,
                                   plan->rt_u32);
     } else {
         M64Rt &rt = plan->rt_m64;
@@ -452,7 +451,7 @@ __global__ void expand_twiddles_kernel(typename FP::T *out, const typename FP::T
     if constexpr (std::is_same_v<FP, FieldGL>) {
         x = gl_canon(gl_mul(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)]));
     } else if constexpr (std::is_same_v<FP, FieldU32>) {
-        x = u32_mul(hi[i >> log_b].w, lo[i & ((1ull << log_b) - 1)], rt.p);
+        x = u32_mont(u32_mont(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)], rt.p, rt.pinv), 1, rt.p, rt.pinv);
     } else {  // Montgomery form product, then back to normal form
         x = m64_mul(m64_mul(hi[i >> log_b], lo[i & ((1ull << log_b) - 1)], rt.p, rt.pinv), 1, rt.p, rt.pinv);
     }
@@ -468,6 +467,9 @@ static int check_u32_params(uint32_t log2n, uint64_t root, uint64_t mod, int dir
         return fail(STNTT_EINVAL, "mod %llu outside [2, 2^31)", (unsigned long long)mod);
     if (dir != STNTT_FORWARD && dir != STNTT_INVERSE) return fail(STNTT_EINVAL, "bad direction %d", dir);
     if (log2n >= 1) {
+        // twiddles are Montgomery words (an odd modulus); an even one would
+        // admit the forward transform only, n = 2^k having no inverse
+        if (!(mod & 1)) return fail(STNTT_EINVAL, "mod %llu must be odd", (unsigned long long)mod);
         uint64_t half = powmod(root, 1ull << (log2n - 1), mod);
         if (half != mod - 1)
             return fail(STNTT_EINVAL, "root %llu is not of order n = 2^%u mod %llu (root^(n/2) != -1)",
@@ -1299,7 +1301,7 @@ int stntt_twiddles_u32(stntt_ctx *ctx, uint64_t n, uint32_t root, uint32_t mod, 
     uint64_t count = n / 2;
     if ((rc = ensure_buffer(&ctx->staging, &ctx->staging_bytes, count * 4))) return rc;
     hipLaunchKernelGGL(expand_twiddles_kernel<FieldU32>, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0,
-                       ctx->stream, (uint32_t *)ctx->staging, (const U32Tw *)plan->hi(), (const U32Tw *)plan->lo(),
+                       ctx->stream, (uint32_t *)ctx->staging, (const uint32_t *)plan->hi(), (const uint32_t *)plan->lo(),
                        L - plan->log_rmax, count, plan->rt_u32);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(host_out, ctx->staging, count * 4, hipMemcpyDeviceToHost, ctx->stream));

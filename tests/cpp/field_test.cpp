@@ -74,10 +74,13 @@ int main() {
         CHECK(gl_sub_mul_pow2<96>(a, b) == ref_pow2(d, 96), "submul96");
         CHECK(gl_sub_mul_pow2<0>(a, b) == d, "submul0");
     }
-    // 32-bit field: Shoup and Barrett products, add, sub, input reduction, for the
-    // findParams primes and the largest prime below 2^31
+    // 32-bit field: Montgomery and Barrett products, add, sub, input reduction,
+    // for the findParams primes and the largest prime below 2^31
     for (uint32_t p : {11u, 12289u, 65537u, 7340033u, 2013265921u, 2147483629u}) {
         uint32_t bar = (uint32_t)((1ull << 32) / p), c32 = (uint32_t)((1ull << 32) % p);
+        const uint32_t pinv = u32_neg_inverse(p);
+        CHECK(p * pinv == 0xFFFFFFFFu, "u32 -p^-1 for %u", p);
+        CHECK(u32_to_mont(1, p) == c32, "u32 Montgomery one for %u", p);
         std::vector<uint32_t> xs = {0, 1, p - 1, p / 2, 0xFFFFFFFFu, 0x80000000u, 0x7FFFFFFFu};
         for (int i = 0; i < 400; ++i) xs.push_back((uint32_t)rng());
         for (uint32_t x : xs) {
@@ -90,8 +93,11 @@ int main() {
                 uint32_t a = x % p, b = y % p;
                 CHECK(u32_add(a, b, p) == (uint32_t)(((uint64_t)a + b) % p), "u32 add");
                 CHECK(u32_sub(a, b, p) == (uint32_t)(((uint64_t)a + p - b) % p), "u32 sub");
-                U32Tw w{b, (uint32_t)(((uint64_t)b << 32) / p)};
-                CHECK(u32_mul(x, w, p) == (uint32_t)((uint64_t)x * b % p), "u32 mul %u * %u mod %u", x, b, p);
+                // any 32-bit data word times a Montgomery twiddle; twiddle times twiddle
+                const uint32_t bm = u32_to_mont(b, p), am = u32_to_mont(a, p);
+                CHECK(u32_mont(x, bm, p, pinv) == (uint32_t)((uint64_t)x * b % p), "u32 mont %u * %u mod %u", x, b, p);
+                CHECK(u32_mont(am, bm, p, pinv) == u32_to_mont((uint64_t)a * b % p, p), "u32 mont twiddle product");
+                CHECK(u32_mont(bm, 1, p, pinv) == b, "u32 mont to normal form");
                 CHECK(u32_mul_any(a, b, p, ~0ull / p) == (uint32_t)((uint64_t)a * b % p), "u32 mul_any %u * %u mod %u",
                       a, b, p);
             }

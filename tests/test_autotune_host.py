@@ -1,0 +1,39 @@
+"""Host logic of the autotuner: candidate splits and table merging (no GPU)."""
+import json
+import os
+
+from stntt_amd import autotune as A
+
+
+def test_pass_candidates_cover_every_stage():
+    for L, logc in ((14, 5), (20, 5), (24, 4), (26, 5)):
+        cands = A.pass_candidates(L, logc)
+        assert cands, L
+        for c in cands:
+            assert sum(c) == L and all(4 <= p <= 9 for p in c)
+            # every pass after the first starts at J >= the tile's columns
+            assert all(sum(c[:i]) >= logc for i in range(1, len(c)))
+        assert len(cands[0]) <= len(cands[-1])  # fewest passes first
+
+
+def test_merge_keeps_entries_not_retuned():
+    old = {"device": "gfx950", "pass_split": {"u32": {"20": [6, 7, 7], "24": [6, 9, 9]}, "goldilocks": {"24": [9, 8, 7]}},
+           "batch_split": {"u32": {"12": [4, 4, 4]}}, "measured_us": {"pass": {"u32": {"20": {"[6, 7, 7]": 18.0}}}}}
+    new = {"device": "gfx950", "pass_split": {"u32": {"24": [8, 8, 8]}, "goldilocks": {}},
+           "batch_split": {"u32": {}, "goldilocks": {}}, "measured_us": {"pass": {"u32": {"24": {"[8, 8, 8]": 110.0}}}}}
+    m = A.merge(old, new)
+    assert m["pass_split"]["u32"] == {"20": [6, 7, 7], "24": [8, 8, 8]}
+    assert m["pass_split"]["goldilocks"] == {"24": [9, 8, 7]}
+    assert m["batch_split"]["u32"] == {"12": [4, 4, 4]}
+    assert m["measured_us"]["pass"]["u32"]["24"] == {"[8, 8, 8]": 110.0}
+    assert old["pass_split"]["u32"]["24"] == [6, 9, 9]  # inputs untouched
+
+
+def test_shipped_table_is_valid():
+    with open(A.TABLE) as f:
+        table = json.load(f)
+    for field, entries in table["pass_split"].items():
+        logc = 4 if field == "goldilocks" else 5
+        for L, split in entries.items():
+            assert sum(split) == int(L) and all(sum(split[:i]) >= logc for i in range(1, len(split))), (field, L)
+    assert os.path.basename(A.TABLE) == "gfx950.json"

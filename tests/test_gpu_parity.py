@@ -239,6 +239,12 @@ def test_errors(ctx):
         S.ntt_stockham(np.zeros(16, dtype=np.int32), 16, 2, 17, ctx=ctx)  # 2 is not of order 16 mod 17
     with pytest.raises(ValueError):
         S.ntt_stockham(np.zeros(16, dtype=np.int32), 16, 3, 2**31 + 11, ctx=ctx)  # modulus too large
+    # 9 has order 2 mod 10 (9 = -1), but twiddles are Montgomery words: odd moduli only
+    with pytest.raises(ValueError):
+        S.ntt_stockham(np.zeros(2, dtype=np.int32), 2, 9, 10, ctx=ctx)
+    w = np.arange(1, 2, dtype=np.int32)
+    S.ntt_stockham(w, 1, 9, 10, ctx=ctx)                  # n = 1: the identity, any modulus
+    assert w[0] == 1
 
 
 @pytest.mark.parametrize("field,L,split", [("goldilocks", 24, [9, 8, 7]), ("goldilocks", 16, [6, 5, 5]),
