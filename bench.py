@@ -54,7 +54,7 @@ def parse():
                     help="also run a plain device copy of the same buffer (counter calibration)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--legs", default="n20,orders,field64,batched,serving",
+    ap.add_argument("--legs", default="n20,u32large,orders,field64,batched,serving",
                     help="comma-separated extra legs to run after the headline")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched leg without the per-transform plain comparison")
@@ -200,6 +200,25 @@ def bench_orders_polymul(S, ctx, steps: int) -> dict:
             "note": "times exclude the two input-restoring device copies timed separately"}
         for d in (da, db, ra, rb):
             d.free()
+    return out
+
+
+def bench_u32_large(S, ctx, steps: int) -> dict:
+    """The reference's 32-bit field at 2^24 and 2^26 (findParams(n, 11)
+    primes), forward and inverse, device-resident; round trip checked."""
+    out = {}
+    for L in (24, 26):
+        n = 1 << L
+        root, mod = S.find_params(n, 11)
+        x = np.random.default_rng(L).integers(0, mod, size=n, dtype=np.uint32)
+        buf = ctx.upload(x)
+        fw = _time_steps(ctx, lambda: ctx.ntt_u32(buf, L, 1, root, mod), steps)
+        iv = _time_steps(ctx, lambda: ctx.ntt_u32(buf, L, 1, root, mod, inverse=True), steps)
+        # as many inverses as forwards ran (steps + 3 each): x is back
+        ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
+        out[f"2^{L}"] = {"mod": mod, "forward_us": round(fw * 1e6, 2), "inverse_us": round(iv * 1e6, 2),
+                         "gelem_s": round(n / fw / 1e9, 2), "roundtrip_exact": bool(ok)}
+        buf.free()
     return out
 
 
@@ -450,6 +469,9 @@ def main() -> int:
     if "orders" in legs:
         log("[bench] element orders and polynomial multiply")
         out["orders_polymul"] = bench_orders_polymul(S, ctx, 20)
+    if "u32large" in legs:
+        log("[bench] 32-bit field at 2^24 and 2^26")
+        out["u32_large"] = bench_u32_large(S, ctx, 20)
     if "field64" in legs:
         log("[bench] 64-bit prime field")
         out["field64_2^24"] = bench_field64(S, ctx, 20)

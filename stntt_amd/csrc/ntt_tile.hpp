@@ -69,18 +69,32 @@ constexpr int tile_threads() {
 // contiguous chunk per column, written back through LDS.
 enum : int { PERM_NONE = 0, PERM_IN = 1, PERM_OUT = 2 };
 
-// Column halves.  A radix-512 tile is 64 KiB of LDS, which holds a CU to two
-// workgroups.  For odd LOGR, r1 = 2 r2 and sub-step 2 has two groups per
-// thread anyway, so the exchange can run in two rounds of C/2 columns
-// through a 32 KiB tile: sub-step 1 keeps its outputs in registers, the
-// threads of one column half write them, every thread runs one sub-step 2
-// group of that half, then the other half follows.  Two more barriers, and
-// twice the resident workgroups.  Not for the permuted variants, which stage
-// the whole tile.
+// Exchange halves.  A radix-512 tile is 64 KiB of LDS, which holds a CU to
+// two workgroups.  For odd LOGR, r1 = 2 r2 and sub-step 2 has two groups per
+// thread anyway, so the exchange can run in two rounds through a 32 KiB
+// image: sub-step 1 keeps its outputs in registers, half of them go to LDS,
+// every thread runs one sub-step 2 group of that half, then the other half
+// follows.  Two more barriers, and twice the resident workgroups.  The
+// split follows the stores: the first pass (whose threads write a column's
+// outputs contiguously) splits the columns; later passes (whose threads
+// write rows of C contiguous words) split the sub-step 2 groups g2, i.e. the
+// rows, so each round still writes whole 128-byte rows.  Splitting their
+// columns instead left every output row as two 64-byte halves written a
+// round apart, and 13-22 % of the lines reached HBM twice.  Not for the
+// permuted variants, which stage the whole tile.
 template <class FP, int LOGR, int PERM>
 constexpr bool tile_half() {
     constexpr int R = 1 << LOGR, C = 1 << tile_logc<FP>();
     return PERM == PERM_NONE && (size_t)R * C * sizeof(typename FP::T) > 32768 && (LOGR & 1);
+}
+
+// Waves per SIMD the register allocator must leave room for (1: no bound).
+// Row halves hold half of sub-step 1's outputs through a sub-step 2 round;
+// for the 64-bit Montgomery field that took radix 512 from 162 to 190
+// registers (two waves per SIMD); bounded to three waves it fits in 168.
+template <class FP, int LOGR, bool J1, int PERM>
+constexpr int tile_min_waves() {
+    return tile_half<FP, LOGR, PERM>() && !J1 ? 3 : 1;
 }
 
 template <class FP, int LOGR, int DIR, bool J1, int NT, int PERM = PERM_NONE>
@@ -100,8 +114,11 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     static_assert(PERM != PERM_IN || (J1 && G1 == 1), "bit-reversed input: first pass, one group per thread");
     static_assert(PERM != PERM_OUT || !J1, "bit-reversed output: last pass only");
     constexpr bool HALF = tile_half<FP, LOGR, PERM>();
-    static_assert(!HALF || (NT == tile_threads<FP, LOGR>() && G1 == 1 && NG2 == 2 * NT), "column halves layout");
-    constexpr int CL = HALF ? C / 2 : C;  // columns of the LDS image
+    static_assert(!HALF || (NT == tile_threads<FP, LOGR>() && G1 == 1 && NG2 == 2 * NT), "exchange halves layout");
+    // halves by columns (first pass) or by rows (later passes)
+    constexpr bool HALF_COLS = HALF && J1, HALF_ROWS = HALF && !J1;
+    constexpr int CL = HALF_COLS ? C / 2 : C;  // columns of the LDS image
+    constexpr int RS = HALF_ROWS ? r1 / 2 : r1;  // image rows between the m-th inputs of a sub-step 2 group
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
@@ -212,6 +229,11 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 T x = v[bitrev_c(k, LR1)];
                 if (k != 0) x = FP::mul(x, w1[k], a.rt);
                 if constexpr (HALF) {
+                    // row halves: computed here, not next to round 1's LDS
+                    // write; sunk past the barrier, each output kept its
+                    // operand and its twiddle live (156 -> 264 registers for
+                    // Goldilocks radix 512)
+                    if constexpr (HALF_ROWS) asm volatile("" : "+v"(x));
                     xo[k] = x;
                 } else {
                     const uint32_t row = r1 * g + k;
@@ -229,8 +251,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     // PERM_OUT: outputs are kept in registers until the LDS tile is free
     [[maybe_unused]] T xs[PERM == PERM_OUT ? G2 : 1][PERM == PERM_OUT ? r2 : 1];
     [[maybe_unused]] uint32_t xc[G2], xg2[G2];
-    // one sub-step 2 group: column c (lc within the LDS image), group g2
-    auto substep2 = [&](uint32_t c, uint32_t g2, uint32_t lc, int gi) {
+    // one sub-step 2 group: column c, group g2; within the LDS image column
+    // lc, group g2l (input m at image row g2l + m RS)
+    auto substep2 = [&](uint32_t c, uint32_t g2, uint32_t lc, uint32_t g2l, int gi) {
         // J = 1: output k = k2 r1 + g2 of column s = t0 + c takes w^(s k),
         // walked as the running product w^(s g2) (w^(s r1))^k2.  Twiddle
         // products are exact in every field (Goldilocks words, Montgomery
@@ -249,7 +272,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         T v[r2];
 #pragma unroll
         for (int m = 0; m < r2; ++m) {
-            const uint32_t row = g2 + m * r1;
+            const uint32_t row = g2l + m * RS;
             v[m] = lds[row * CL + (lc ^ (row & (CL - 1)))];
         }
         dft_regs<FP, LR2, DIR>(v, roots2, a.rt);
@@ -282,7 +305,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         xc[gi] = c;
         xg2[gi] = g2;
     };
-    if constexpr (HALF) {
+    if constexpr (HALF_COLS) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if ((xo_c >> (LOGC - 1)) == (uint32_t)h) {  // this thread's column is in half h
@@ -293,15 +316,25 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 }
             }
             __syncthreads();
-            uint32_t lc, g2;
-            if constexpr (J1) {  // lanes walk g2
-                g2 = tid & (r1 - 1);
-                lc = tid >> LR1;
-            } else {  // lanes walk c
-                lc = tid & (CL - 1);
-                g2 = tid >> (LOGC - 1);
+            // lanes walk g2: a column's outputs are contiguous
+            const uint32_t g2 = tid & (r1 - 1), lc = tid >> LR1;
+            substep2(h * CL + lc, g2, lc, g2, h);
+            if (h == 0) __syncthreads();  // round 0's reads before round 1's writes
+        }
+    } else if constexpr (HALF_ROWS) {
+        // round h: groups g2 in [h r1/2, (h+1) r1/2), i.e. sub-step 1 outputs
+        // k in that range; output k of group g sits at image row g r1/2 + k'
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int k = 0; k < r1 / 2; ++k) {
+                const uint32_t row = (r1 / 2) * xo_g + k;
+                lds[row * C + (xo_c ^ (row & (C - 1)))] = xo[h * (r1 / 2) + k];
             }
-            substep2(h * CL + lc, g2, lc, h);
+            __syncthreads();
+            // lanes walk c: a row of C outputs is contiguous
+            const uint32_t c = tid & (C - 1), g2l = tid >> LOGC;
+            substep2(c, h * (r1 / 2) + g2l, c, g2l, h);
             if (h == 0) __syncthreads();  // round 0's reads before round 1's writes
         }
     } else {
@@ -310,9 +343,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             const uint32_t q = tid + gi * NT;
             if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) continue;
             if constexpr (J1) {  // outputs of a column are contiguous: lanes walk g2
-                substep2(q >> LR1, q & (r1 - 1), q >> LR1, gi);
+                substep2(q >> LR1, q & (r1 - 1), q >> LR1, q & (r1 - 1), gi);
             } else {  // outputs of a row are contiguous: lanes walk c
-                substep2(q & (C - 1), q >> LOGC, q & (C - 1), gi);
+                substep2(q & (C - 1), q >> LOGC, q & (C - 1), q >> LOGC, gi);
             }
         }
     }
@@ -341,7 +374,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 }
 
 template <class FP, int LOGR, int DIR, bool J1, int PERM = PERM_NONE>
-__global__ void __launch_bounds__((tile_threads<FP, LOGR>())) ntt_tile_kernel(PassArgs<FP> a) {
+__global__ void __launch_bounds__((tile_threads<FP, LOGR>()), (tile_min_waves<FP, LOGR, J1, PERM>()))
+    ntt_tile_kernel(PassArgs<FP> a) {
     constexpr int C = 1 << tile_logc<FP>();
     constexpr int R = 1 << LOGR;
     __shared__ typename FP::T lds[R * C / (tile_half<FP, LOGR, PERM>() ? 2 : 1)];
