@@ -51,8 +51,8 @@ __device__ __forceinline__ typename FP::T batch_twiddle_mul(typename FP::T x, ui
     const typename FP::Tw *hi = (const typename FP::Tw *)t.hi;
     const typename FP::Tw *lo = (const typename FP::Tw *)t.lo;
     const uint32_t log_b = t.log_n - t.log_rmax;
-    x = FP::mul(x, hi[e >> log_b], rt);
-    if (log_b) x = FP::mul(x, lo[e & ((1u << log_b) - 1)], rt);
+    x = FP::mul(x, at_off(hi, e >> log_b), rt);
+    if (log_b) x = FP::mul(x, at_off(lo, e & ((1u << log_b) - 1)), rt);
     return x;
 }
 
@@ -72,7 +72,8 @@ __device__ __forceinline__ typename FP::T batch_out(typename FP::T x, uint32_t s
 // n = 2^L points, in place: every thread reads all its groups, the
 // workgroup synchronises, then writes.  Inner J' = 2^logj.
 template <class FP, int LR, int DIR, int GMAX>
-__device__ __forceinline__ void batch_substep_lds(typename FP::T *z, uint32_t L, uint32_t logj, const BatchTask &t,
+__device__ __forceinline__ void batch_substep_lds(shared_t<typename FP::T> *z, uint32_t L, uint32_t logj,
+                                                  const BatchTask &t,
                                                   const typename FP::Rt &rt, uint32_t flags, bool last) {
     using T = typename FP::T;
     constexpr int r = 1 << LR;
@@ -104,7 +105,8 @@ __device__ __forceinline__ void batch_substep_lds(typename FP::T *z, uint32_t L,
 // The same sub-step streaming from `in` to a distinct `out` in global
 // memory, one group at a time.
 template <class FP, int LR, int DIR>
-__device__ __forceinline__ void batch_substep_global(const typename FP::T *in, typename FP::T *out, uint32_t L,
+__device__ __forceinline__ void batch_substep_global(const global_t<typename FP::T> *in,
+                                                     global_t<typename FP::T> *out, uint32_t L,
                                                      uint32_t logj, const BatchTask &t, const typename FP::Rt &rt,
                                                      uint32_t flags, bool last) {
     using T = typename FP::T;
@@ -124,7 +126,7 @@ __device__ __forceinline__ void batch_substep_global(const typename FP::T *in, t
 }
 
 template <class FP, int DIR>
-__device__ void batch_run_task(const BatchTask &t, typename FP::T *scratch, unsigned char *smem) {
+__device__ __forceinline__ void batch_run_task(const BatchTask &t, typename FP::T *scratch, unsigned char *smem) {
     using T = typename FP::T;
     const typename FP::Rt rt = [&] {
         if constexpr (std::is_same_v<FP, FieldGL>) return t.rtgl;
@@ -132,15 +134,18 @@ __device__ void batch_run_task(const BatchTask &t, typename FP::T *scratch, unsi
     }();
     const uint32_t L = t.log_n;
     const uint32_t n = 1u << L;
-    const T *src = (const T *)t.src;
-    T *dst = (T *)t.dst;
+    // descriptor pointers are device memory: global_t keeps the accesses
+    // global (not flat) so LDS waits do not also wait for HBM stores
+    const global_t<T> *src = (const global_t<T> *)t.src;
+    global_t<T> *dst = (global_t<T> *)t.dst;
+    global_t<T> *gscratch = (global_t<T> *)scratch;
     if (L == 0) {  // zero stages: identity
         if (threadIdx.x == 0) dst[0] = src[0];
         return;
     }
     if (n * sizeof(T) <= (uint32_t)BATCH_LDS_BYTES) {
         // ---- whole transform in LDS ----
-        T *z = (T *)smem;
+        shared_t<T> *z = (shared_t<T> *)smem;  // LDS accesses, not flat
         for (uint32_t i = threadIdx.x; i < n; i += BATCH_THREADS) z[i] = FP::load_in(src[i], rt, t.flags);
         __syncthreads();
         uint32_t logj = 0;
@@ -165,11 +170,11 @@ __device__ void batch_run_task(const BatchTask &t, typename FP::T *scratch, unsi
         // buffers alternate so that the last sub-step writes dst; the first
         // reads src (which may equal dst only when the count is even)
         uint32_t logj = 0;
-        const T *in = src;
+        const global_t<T> *in = src;
         for (uint32_t i = 0; i < t.nsub; ++i) {
             const int lr = (t.radix_logs >> (4 * i)) & 15;
             const bool last = i + 1 == t.nsub;
-            T *out = ((t.nsub - 1 - i) & 1) ? scratch : dst;
+            global_t<T> *out = ((t.nsub - 1 - i) & 1) ? gscratch : dst;
             const uint32_t fl = i == 0 ? t.flags : (t.flags & ~PASS_FIRST);
             switch (lr) {
                 case 4: batch_substep_global<FP, 4, DIR>(in, out, L, logj, t, rt, fl, last); break;
@@ -203,10 +208,12 @@ __global__ void __launch_bounds__(BATCH_THREADS) ntt_batch_kernel(const BatchTas
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
         __syncthreads();
-        const uint32_t idx = next;
+        // uniform: the descriptor loads into scalar registers, once (a
+        // reference into global memory is re-read after every store)
+        const uint32_t idx = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
         if (idx >= count) break;
-        const BatchTask &t = tasks[idx];
+        const BatchTask t = tasks[idx];
         if (t.field == 1) {
             if (t.dir == 0) batch_run_task<FieldGL, 0>(t, my_scratch, smem);
             else batch_run_task<FieldGL, 1>(t, my_scratch, smem);

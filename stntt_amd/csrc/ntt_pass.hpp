@@ -74,6 +74,29 @@ enum : uint32_t {
     PASS_BITREV_OUT = 32u,  // output in bit-reversed order (single-pass kernel)
 };
 
+// Global (device) memory.  A generic pointer the compiler cannot trace to a
+// kernel argument -- one read from a pointer table, or chosen between a
+// table entry and an argument -- compiles to flat_load / flat_store, which
+// count against the LDS wait counter too: every LDS wait and barrier of the
+// pass then also waited for the outstanding HBM stores.
+template <class T>
+using global_t = __attribute__((address_space(1))) T;
+// LDS, for a shared array reached through a function parameter
+template <class T>
+using shared_t = __attribute__((address_space(3))) T;
+
+// element `elem` (a 32-bit index: n <= 2^32) of a uniform base pointer into
+// global memory (data buffers and twiddle tables, never LDS); the byte
+// offset is formed in 64 bits (8-byte words past 2^29 need 33 bits)
+template <class T>
+__device__ __forceinline__ const global_t<T> &at_off(const T *base, uint32_t elem) {
+    return *(const global_t<T> *)((const char *)base + (uint64_t)elem * sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ global_t<T> &at_off(T *base, uint32_t elem) {
+    return *(global_t<T> *)((char *)base + (uint64_t)elem * sizeof(T));
+}
+
 // bit reversal of the low `bits` bits of x, 1 <= bits <= 32
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits) { return __brev(x) >> (32 - bits); }
 
@@ -85,6 +108,8 @@ struct FieldGL {
     };
     // runtime roots are not needed: omega_r^j is a power of two
     struct Roots {};
+    // a general product is ~25 VALU: inner twiddles come from the table
+    static constexpr bool CHEAP_TWIDDLE_PRODUCT = false;
     static __device__ __forceinline__ T add(T a, T b, const Rt &) { return gl_add(a, b); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &) { return gl_sub(a, b); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &) { return gl_mul(a, w); }
@@ -127,6 +152,9 @@ struct FieldU32 {
     struct Roots {
         Tw w[16];  // omega_r^j, j < r/2 (r <= 32)
     };
+    // a twiddle product (5 VALU) is cheaper than holding a gathered table
+    // word per power in a register
+    static constexpr bool CHEAP_TWIDDLE_PRODUCT = true;
 
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32_sub(a, b, rt.p); }
@@ -138,7 +166,7 @@ struct FieldU32 {
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
         Roots r;
 #pragma unroll
-        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = hi[j << (log_rmax - LR)];
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = at_off(hi, (uint32_t)(j << (log_rmax - LR)));
         return r;
     }
     template <int LR, int DIR, int J>
@@ -168,6 +196,7 @@ struct FieldM64 {
     struct Roots {
         Tw w[16];  // omega_r^j (Montgomery form), j < r/2 (r <= 32)
     };
+    static constexpr bool CHEAP_TWIDDLE_PRODUCT = false;
     static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return m64_add(a, b, rt.p); }
     static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return m64_sub(a, b, rt.p); }
     static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return m64_mul(a, w, rt.p, rt.pinv); }
@@ -179,7 +208,7 @@ struct FieldM64 {
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
         Roots r;
 #pragma unroll
-        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = hi[j << (log_rmax - LR)];
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = at_off(hi, (uint32_t)(j << (log_rmax - LR)));
         return r;
     }
     template <int LR, int DIR, int J>
@@ -358,7 +387,7 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
         if ((uint32_t)c < ncols) {
             const uint64_t gi = in_base + c * cs_in + m * ms_in;
             T raw;
-            if constexpr (SINGLE) raw = a.in_tab ? a.in_tab[(uint64_t)blockIdx.x * C + c][m] : a.in[gi];
+            if constexpr (SINGLE) raw = a.in_tab ? at_off(a.in_tab[(uint64_t)blockIdx.x * C + c], (uint32_t)m) : a.in[gi];
             else raw = a.in[gi];
             x = FP::load_in(raw, a.rt, a.flags);
             if (a.mul_in) x = FP::mulg(x, a.mul_in[gi], a.rt);
@@ -401,7 +430,7 @@ __global__ void __launch_bounds__(NT) ntt_pass_kernel(PassArgs<FP> a) {
         if (a.flags & PASS_LAST) x = FP::finish(x, a.rt, a.flags);
         if constexpr (SINGLE) {
             if (a.out_tab) {
-                a.out_tab[(uint64_t)blockIdx.x * C + c][k] = x;
+                at_off(a.out_tab[(uint64_t)blockIdx.x * C + c], (uint32_t)k) = x;
                 continue;
             }
         }

@@ -26,17 +26,6 @@
 
 namespace stntt {
 
-// element `elem` (a 32-bit index: n <= 2^32) of a uniform base pointer; the
-// byte offset is formed in 64 bits (8-byte words past 2^29 need 33 bits)
-template <class T>
-__device__ __forceinline__ const T &at_off(const T *base, uint32_t elem) {
-    return *(const T *)((const char *)base + (uint64_t)elem * sizeof(T));
-}
-template <class T>
-__device__ __forceinline__ T &at_off(T *base, uint32_t elem) {
-    return *(T *)((char *)base + (uint64_t)elem * sizeof(T));
-}
-
 template <class FP>
 constexpr int tile_logc() {
     return sizeof(typename FP::T) == 8 ? 4 : 5;  // 128-byte rows
@@ -193,8 +182,12 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 #pragma unroll
                 for (int m = 0; m < r1; ++m) v[m] = at_off(in + (uint64_t)m * mstep, lane);
             }
+            if constexpr (FP::CHEAP_TWIDDLE_PRODUCT) {
+                w1[1] = at_off(a.hi, tstep);  // the rest follow as its powers
+            } else {
 #pragma unroll
-            for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
+                for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
+            }
         }
         // keep the scheduler from pulling a dependent use (and its wait)
         // in between the loads
@@ -224,10 +217,16 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 for (int m = 0; m < r1; ++m) v[m] = FP::load_in(v[m], a.rt, a.flags);
             }
             dft_regs<FP, LR1, DIR>(v, roots, a.rt);
+            [[maybe_unused]] Tw wk = w1[1];  // w_R^(g k), walked along k
 #pragma unroll
             for (int k = 0; k < r1; ++k) {
                 T x = v[bitrev_c(k, LR1)];
-                if (k != 0) x = FP::mul(x, w1[k], a.rt);
+                if constexpr (FP::CHEAP_TWIDDLE_PRODUCT) {
+                    if (k >= 2) wk = FP::twiddle(wk, w1[1], a.rt);
+                    if (k != 0) x = FP::mul(x, wk, a.rt);
+                } else {
+                    if (k != 0) x = FP::mul(x, w1[k], a.rt);
+                }
                 if constexpr (HALF) {
                     // row halves: computed here, not next to round 1's LDS
                     // write; sunk past the barrier, each output kept its
