@@ -4,7 +4,8 @@
  * Forward and inverse number-theoretic transforms, natural order in and out,
  * over
  *   - the reference's 32-bit prime field: runtime (root, mod) as produced by
- *     findParams(n, min_mod), mod < 2^31 prime, root of order exactly n;
+ *     findParams(n, min_mod), mod < 2^31 prime (odd when n >= 2: twiddles
+ *     are Montgomery words), root of order exactly n;
  *   - Goldilocks, P = 2^64 - 2^32 + 1, root 7^((P-1)/n).
  * Buffers are contiguous [batch][n] rows, transformed in place from the
  * caller's point of view (src/ntt/ntt_cpu.cpp:87-90,

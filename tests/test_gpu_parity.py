@@ -226,6 +226,37 @@ def test_u32_large(ctx, L):
         assert np.array_equal(X, O.stockham_u32(x, n, root, mod))
 
 
+# The 32-bit field's twiddles are Montgomery words (REDC with -p^-1 mod 2^32):
+# moduli at both ends of its range, through every kernel family (single-pass,
+# tile passes, the batch queue).  root = None: findParams from that modulus up
+# (each listed modulus is the prime it finds).
+EXTREME_MODULI = [(1, 3, 2), (2, 5, 2), (4, 17, 3), (1, 2**31 - 1, 2**31 - 2),
+                  (10, 2145389569, None), (14, 2013265921, None), (16, 2130706433, None),
+                  (20, 1888485377, None)]
+
+
+@pytest.mark.parametrize("L,mod,root", EXTREME_MODULI)
+def test_u32_extreme_moduli(ctx, L, mod, root):
+    n = 1 << L
+    if root is None:
+        root, found = S.find_params(n, mod)
+        assert found == mod
+    batches = 3
+    x = u32_input(7000 + L, n * batches, mod)
+    X = run_u32(ctx, x, L, batches, root, mod)
+    assert np.array_equal(X, O.stockham_u32(x, n, root, mod, batches)), f"forward mod={mod}"
+    assert np.array_equal(run_u32(ctx, X, L, batches, root, mod, inverse=True), x), f"round trip mod={mod}"
+    # the same transforms through the batch queue / grouped passes
+    bufs = [ctx.upload(x[i * n:(i + 1) * n].copy()) for i in range(batches)]
+    b = S.Batch(ctx, [("u32", False, L, buf, root, mod) for buf in bufs])
+    b.run()
+    ctx.sync()
+    for i, buf in enumerate(bufs):
+        assert np.array_equal(buf.copy_to(np.empty(n, dtype=np.uint32)), X[i * n:(i + 1) * n]), f"batch mod={mod}"
+        buf.free()
+    b.close()
+
+
 # ----------------------------------------------------------------------------
 # error behaviour of the reference's API
 # ----------------------------------------------------------------------------
