@@ -67,7 +67,21 @@ GL_HD uint64_t gl_reduce128_w(uint64_t lo, uint64_t hi) {
     return t;
 }
 
-GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) { return gl_canon(gl_reduce128_w(lo, hi)); }
+// The same with a canonical result, ordered for fewer instructions:
+//   s = lo - h1                 a borrow is worth -EPS (s + P, no second
+//                               borrow: lo - h1 + 2^64 > EPS)
+//   r = h0 * EPS + s            one v_mad_u64_u32; s is any 64-bit word
+// A carry of r is worth +EPS, and then r + EPS <= 2^64 - 2^32 - 1 < P is
+// already canonical; without one, r - P = r + EPS (mod 2^64) is due exactly
+// when r + EPS wraps.  Both corrections are the one word r + EPS.
+GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
+    const uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
+    uint64_t s;
+    if (__builtin_sub_overflow(lo, h1, &s)) s -= GL_EPS;
+    const uint64_t r = h0 * GL_EPS + s;
+    const uint64_t t = r + GL_EPS;
+    return (r < s || t < r) ? t : r;
+}
 
 // 64 x 64 -> 128 from four 32 x 32 -> 64 multiply-adds
 GL_HD void gl_mul_wide(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
@@ -124,7 +138,7 @@ GL_HD uint64_t gl_mul_pow2(uint64_t x) {
     uint64_t r;
     if constexpr (s == 0) r = gl_canon(x);
     else if constexpr (s <= 32) r = gl_shl_small(x, s);
-    else if constexpr (s < 64) r = gl_canon(gl_shl_w(x, s));
+    else if constexpr (s < 64) r = gl_reduce128(x << s, x >> (64 - s));
     else r = gl_shl_small(gl_shl_w(x, s - 32), 32);
     if constexpr (neg) return gl_sub(0, r);
     return r;
