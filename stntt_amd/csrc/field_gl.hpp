@@ -43,13 +43,23 @@ GL_HD uint64_t gl_add(uint64_t a, uint64_t b) {
     return (s < a || t < s) ? t : s;
 }
 
+// d = a - b (mod 2^64) as a two-word borrow chain; returns the borrow.  The
+// chain's own borrow-out (VCC after v_subb_co_u32) selects the correction;
+// a 64-bit __builtin_sub_overflow compiled to the same subtraction plus a
+// 64-bit compare that recomputed it.
+GL_HD bool gl_sub_borrow(uint64_t a, uint64_t b, uint64_t &d) {
+    unsigned c1, c2;
+    const uint32_t lo = __builtin_subc((uint32_t)a, (uint32_t)b, 0u, &c1);
+    const uint32_t hi = __builtin_subc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c1, &c2);
+    d = ((uint64_t)hi << 32) | lo;
+    return c2 != 0;
+}
+
 // a - b, a, b < P.  A borrow means a - b + 2^64; adding P is subtracting
-// EPS, which cannot borrow again (a - b + 2^64 > 2^32).  The borrow flag of
-// the subtraction itself selects the correction.
+// EPS, which cannot borrow again (a - b + 2^64 > 2^32).
 GL_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
     uint64_t d;
-    bool borrow = __builtin_sub_overflow(a, b, &d);
-    return borrow ? d + GL_P : d;
+    return gl_sub_borrow(a, b, d) ? d + GL_P : d;
 }
 
 // lo + hi * 2^64 (mod P) for any lo, hi; result is any 64-bit word
@@ -62,8 +72,8 @@ GL_HD uint64_t gl_reduce128_w(uint64_t lo, uint64_t hi) {
     uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
     uint64_t r = h0 * GL_EPS + lo;
     if (r < lo) r += GL_EPS;
-    uint64_t t = r - h1;
-    if (r < h1) t -= GL_EPS;
+    uint64_t t;
+    if (gl_sub_borrow(r, h1, t)) t -= GL_EPS;
     return t;
 }
 
@@ -77,7 +87,7 @@ GL_HD uint64_t gl_reduce128_w(uint64_t lo, uint64_t hi) {
 GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
     const uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
     uint64_t s;
-    if (__builtin_sub_overflow(lo, h1, &s)) s -= GL_EPS;
+    if (gl_sub_borrow(lo, h1, s)) s -= GL_EPS;
     const uint64_t r = h0 * GL_EPS + s;
     const uint64_t t = r + GL_EPS;
     return (r < s || t < r) ? t : r;
@@ -135,13 +145,24 @@ GL_HD uint64_t gl_mul_pow2(uint64_t x) {
     constexpr int s192 = ((S % 192) + 192) % 192;
     constexpr bool neg = s192 >= 96;
     constexpr int s = s192 % 96;
-    uint64_t r;
-    if constexpr (s == 0) r = gl_canon(x);
-    else if constexpr (s <= 32) r = gl_shl_small(x, s);
-    else if constexpr (s < 64) r = gl_reduce128(x << s, x >> (64 - s));
-    else r = gl_shl_small(gl_shl_w(x, s - 32), 32);
-    if constexpr (neg) return gl_sub(0, r);
-    return r;
+    if constexpr (s >= 64) {
+        // 64 <= s < 96: x 2^s = -x 2^-v with v = 96 - s in (0, 32].  With
+        // x = q 2^v + r (r < 2^v) and w = r 2^(32-v) < 2^32:
+        //   x 2^-v = q + r 2^-v = q - r 2^(96-v) = q - w 2^64 == q - w EPS,
+        // so x 2^s == w EPS - q: a shift, a multiply-add and one
+        // subtraction (q <= 2^63 and w EPS < 2^64 - 2^33 are both < P).
+        constexpr int v = 96 - s;
+        const uint64_t q = x >> v;
+        const uint64_t e = (uint64_t)((uint32_t)x << (32 - v)) * GL_EPS;
+        return neg ? gl_sub(q, e) : gl_sub(e, q);
+    } else {
+        uint64_t r;
+        if constexpr (s == 0) r = gl_canon(x);
+        else if constexpr (s <= 32) r = gl_shl_small(x, s);
+        else r = gl_reduce128(x << s, x >> (64 - s));
+        if constexpr (neg) return gl_sub(0, r);
+        return r;
+    }
 }
 
 // omega_64 = 7^((P-1)/64) = 2^39 (mod P), so every root of unity of order

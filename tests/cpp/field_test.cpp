@@ -72,6 +72,12 @@ int main() {
         CHECK(gl_sub_mul_pow2<120>(a, b) == ref_pow2(d, 120), "submul120");
         CHECK(gl_sub_mul_pow2<36>(a, b) == ref_pow2(d, 36), "submul36");
         CHECK(gl_sub_mul_pow2<96>(a, b) == ref_pow2(d, 96), "submul96");
+        // 64 <= s < 96 directly and after the sign fold (S - 96 in that range)
+        CHECK(gl_sub_mul_pow2<64>(a, b) == ref_pow2(d, 64), "submul64");
+        CHECK(gl_sub_mul_pow2<72>(a, b) == ref_pow2(d, 72), "submul72");
+        CHECK(gl_sub_mul_pow2<95>(a, b) == ref_pow2(d, 95), "submul95");
+        CHECK(gl_sub_mul_pow2<168>(a, b) == ref_pow2(d, 168), "submul168");
+        CHECK(gl_sub_mul_pow2<180>(a, b) == ref_pow2(d, 180), "submul180");
         CHECK(gl_sub_mul_pow2<0>(a, b) == d, "submul0");
     }
     // 32-bit field: Montgomery and Barrett products, add, sub, input reduction,
