@@ -86,9 +86,18 @@ constexpr int tile_min_waves() {
     return tile_half<FP, LOGR, PERM>() && !J1 ? 3 : 1;
 }
 
+// Inner twiddles from a per-tile LDS copy of the R roots w_R^e (exchange
+// halves of fields without a cheap twiddle product): each thread reads its
+// w_R^(g k) when it multiplies, instead of holding r1 - 1 gathered words
+// from the start of the tile.
+template <class FP, int LOGR, int PERM>
+constexpr bool tile_inner_lds() {
+    return tile_half<FP, LOGR, PERM>() && !FP::CHEAP_TWIDDLE_PRODUCT;
+}
+
 template <class FP, int LOGR, int DIR, bool J1, int NT, int PERM = PERM_NONE>
 __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block, typename FP::T *lds,
-                                          typename FP::Tw *twl) {
+                                          typename FP::Tw *twl, typename FP::Tw *twi) {
     using T = typename FP::T;
     using Tw = typename FP::Tw;
     constexpr int LOGC = tile_logc<FP>();
@@ -108,6 +117,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     constexpr bool HALF_COLS = HALF && J1, HALF_ROWS = HALF && !J1;
     constexpr int CL = HALF_COLS ? C / 2 : C;  // columns of the LDS image
     constexpr int RS = HALF_ROWS ? r1 / 2 : r1;  // image rows between the m-th inputs of a sub-step 2 group
+    constexpr bool INNER_LDS = tile_inner_lds<FP, LOGR, PERM>();
+    static_assert(!INNER_LDS || (G1 == 1 && R % NT == 0), "inner twiddle table layout");
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
@@ -184,10 +195,15 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             }
             if constexpr (FP::CHEAP_TWIDDLE_PRODUCT) {
                 w1[1] = at_off(a.hi, tstep);  // the rest follow as its powers
-            } else {
+            } else if constexpr (!INNER_LDS) {
 #pragma unroll
                 for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
             }
+        }
+        [[maybe_unused]] Tw ti[INNER_LDS ? R / NT : 1];
+        if constexpr (INNER_LDS) {
+#pragma unroll
+            for (int i = 0; i < R / NT; ++i) ti[i] = at_off(a.hi, (tid + i * NT) << (a.log_rmax - LOGR));
         }
         // keep the scheduler from pulling a dependent use (and its wait)
         // in between the loads
@@ -210,6 +226,10 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 }
             }
         }
+        if constexpr (INNER_LDS) {
+#pragma unroll
+            for (int i = 0; i < R / NT; ++i) twi[tid + i * NT] = ti[i];
+        }
         if constexpr (PERM == PERM_IN) __syncthreads();  // staged rows all read before reuse
         if (act) {
             if constexpr (PERM != PERM_IN) {
@@ -217,6 +237,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 for (int m = 0; m < r1; ++m) v[m] = FP::load_in(v[m], a.rt, a.flags);
             }
             dft_regs<FP, LR1, DIR>(v, roots, a.rt);
+        }
+        if constexpr (INNER_LDS) __syncthreads();  // the root table is complete
+        if (act) {
             [[maybe_unused]] Tw wk = w1[1];  // w_R^(g k), walked along k
 #pragma unroll
             for (int k = 0; k < r1; ++k) {
@@ -224,6 +247,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 if constexpr (FP::CHEAP_TWIDDLE_PRODUCT) {
                     if (k >= 2) wk = FP::twiddle(wk, w1[1], a.rt);
                     if (k != 0) x = FP::mul(x, wk, a.rt);
+                } else if constexpr (INNER_LDS) {
+                    if (k != 0) x = FP::mul(x, twi[g * k], a.rt);  // g k < R
                 } else {
                     if (k != 0) x = FP::mul(x, w1[k], a.rt);
                 }
@@ -379,7 +404,8 @@ __global__ void __launch_bounds__((tile_threads<FP, LOGR>()), (tile_min_waves<FP
     constexpr int R = 1 << LOGR;
     __shared__ typename FP::T lds[R * C / (tile_half<FP, LOGR, PERM>() ? 2 : 1)];
     __shared__ typename FP::Tw twl[J1 ? 1 : R];
-    tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>(), PERM>(a, blockIdx.x, lds, twl);
+    __shared__ typename FP::Tw twi[tile_inner_lds<FP, LOGR, PERM>() ? R : 1];
+    tile_body<FP, LOGR, DIR, J1, tile_threads<FP, LOGR>(), PERM>(a, blockIdx.x, lds, twl, twi);
 }
 
 }  // namespace stntt
