@@ -20,7 +20,11 @@
  *
  * Threading: a context is externally synchronised (one thread at a time);
  * distinct contexts are independent.  Device entry points are asynchronous on
- * the given stream (0 = the context's own stream).
+ * the given stream (0 = the context's own stream).  Transforms above the
+ * single-pass sizes run through the context's scratch buffer: one issued on
+ * a different stream than the previous such transform first waits for it
+ * (an event), so they are correct on any mix of streams but do not overlap;
+ * use one context per stream for concurrent large transforms.
  */
 #ifndef STNTT_H
 #define STNTT_H

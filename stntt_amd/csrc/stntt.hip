@@ -126,6 +126,11 @@ struct stntt_ctx {
     std::map<stntt::PlanKey, std::unique_ptr<stntt::Plan>> plans;
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // Multi-pass transforms on different streams share `scratch`: each one
+    // records scratch_ev on its stream when its passes are enqueued, and one
+    // on another stream waits for it first.
+    hipEvent_t scratch_ev = nullptr;
+    hipStream_t scratch_last = nullptr;  // stream of that record (compared, never used)
     void *staging = nullptr;  // device buffer of the host-buffer entry points
     size_t staging_bytes = 0;
     // per-pass timing: when on, events bracket every kernel of a transform
@@ -380,6 +385,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
             if (rc) return rc;
             scratch[0] = (T *)ctx->scratch;
+            if (ctx->scratch_last && ctx->scratch_last != s) HIP_TRY(hipStreamWaitEvent(s, ctx->scratch_ev, 0));
         }
         scratch[1] = halves > 1 ? scratch[0] + total : nullptr;
     }
@@ -438,6 +444,10 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
         src = dst;
         log_j += logr;
+    }
+    if (sch.npasses > 1 && !ext_scratch) {
+        HIP_TRY(hipEventRecord(ctx->scratch_ev, s));
+        ctx->scratch_last = s;
     }
     return STNTT_OK;
 }
@@ -1079,6 +1089,7 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
     HIP_TRY(hipEventCreate(&ctx->ev_start));
     HIP_TRY(hipEventCreate(&ctx->ev_stop));
     for (auto &e : ctx->prof_ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming));
     *out = ctx.release();
     return STNTT_OK;
 }
@@ -1086,13 +1097,15 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
 int stntt_ctx_destroy(stntt_ctx *ctx) {
     if (!ctx) return STNTT_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    // transforms on callers' streams may still read the tables and scratch
+    (void)hipDeviceSynchronize();
     for (auto &kv : ctx->plans) (void)hipFree(kv.second->d_tables);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->staging) (void)hipFree(ctx->staging);
     (void)hipEventDestroy(ctx->ev_start);
     (void)hipEventDestroy(ctx->ev_stop);
     for (auto &e : ctx->prof_ev) (void)hipEventDestroy(e);
+    if (ctx->scratch_ev) (void)hipEventDestroy(ctx->scratch_ev);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return STNTT_OK;
