@@ -158,6 +158,10 @@ struct stntt_batch {
     uint32_t grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // runs share the queue counter and the group scratch: a run on another
+    // stream than the previous one waits for it (run_ev, recorded after each)
+    hipEvent_t run_ev = nullptr;
+    hipStream_t run_last = nullptr;  // stream of that record (compared, never used)
 };
 
 namespace stntt {
@@ -165,6 +169,13 @@ namespace stntt {
 static int set_device(stntt_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
     return STNTT_OK;
+}
+
+// true when s is recording a graph: the cross-stream orderings below (event
+// record / wait between eager launches) are skipped for captured work
+static bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
 }
 
 static int ensure_buffer(void **buf, size_t *cap, size_t bytes) {
@@ -385,7 +396,8 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
             if (rc) return rc;
             scratch[0] = (T *)ctx->scratch;
-            if (ctx->scratch_last && ctx->scratch_last != s) HIP_TRY(hipStreamWaitEvent(s, ctx->scratch_ev, 0));
+            if (ctx->scratch_last && ctx->scratch_last != s && !stream_capturing(s))
+                HIP_TRY(hipStreamWaitEvent(s, ctx->scratch_ev, 0));
         }
         scratch[1] = halves > 1 ? scratch[0] + total : nullptr;
     }
@@ -445,7 +457,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         src = dst;
         log_j += logr;
     }
-    if (sch.npasses > 1 && !ext_scratch) {
+    if (sch.npasses > 1 && !ext_scratch && !stream_capturing(s)) {
         HIP_TRY(hipEventRecord(ctx->scratch_ev, s));
         ctx->scratch_last = s;
     }
@@ -696,6 +708,7 @@ static int batch_launch(stntt_batch *b, hipStream_t s) {
 }
 
 static void batch_free(stntt_batch *b) {
+    if (b->run_ev) (void)hipEventDestroy(b->run_ev);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tasks) (void)hipFree(b->d_tasks);
@@ -1396,6 +1409,10 @@ int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
     if ((rc = set_device(ctx))) return rc;
     auto b = std::make_unique<stntt_batch>();
     b->ctx = ctx;
+    if (hipEventCreateWithFlags(&b->run_ev, hipEventDisableTiming) != hipSuccess) {
+        b->run_ev = nullptr;
+        return fail(STNTT_EDEVICE, "hipEventCreateWithFlags failed");
+    }
     if ((rc = batch_build(ctx, descs, count, b.get()))) {
         batch_free(b.get());
         return rc;
@@ -1404,11 +1421,27 @@ int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
     return STNTT_OK;
 }
 
+// order a run of b on s after the previous one (another stream: an event
+// wait), and mark this one's end
+static int batch_order_begin(stntt_batch *b, hipStream_t s) {
+    if (b->run_last && b->run_last != s && !stream_capturing(s)) HIP_TRY(hipStreamWaitEvent(s, b->run_ev, 0));
+    return STNTT_OK;
+}
+static int batch_order_end(stntt_batch *b, hipStream_t s) {
+    if (stream_capturing(s)) return STNTT_OK;
+    HIP_TRY(hipEventRecord(b->run_ev, s));
+    b->run_last = s;
+    return STNTT_OK;
+}
+
 int stntt_batch_run(stntt_batch *b, stntt_stream stream) {
     if (!b) return fail(STNTT_EINVAL, "null batch");
     int rc;
     if ((rc = set_device(b->ctx))) return rc;
-    return batch_launch(b, stream ? (hipStream_t)stream : b->ctx->stream);
+    hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+    if ((rc = batch_order_begin(b, s))) return rc;
+    if ((rc = batch_launch(b, s))) return rc;
+    return batch_order_end(b, s);
 }
 
 int stntt_batch_capture(stntt_batch *b, stntt_stream stream) {
@@ -1436,8 +1469,10 @@ int stntt_batch_run_graph(stntt_batch *b, stntt_stream stream) {
     if (!b->graph_exec) return fail(STNTT_EINVAL, "batch not captured");
     int rc;
     if ((rc = set_device(b->ctx))) return rc;
-    HIP_TRY(hipGraphLaunch(b->graph_exec, stream ? (hipStream_t)stream : b->ctx->stream));
-    return STNTT_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : b->ctx->stream;
+    if ((rc = batch_order_begin(b, s))) return rc;
+    HIP_TRY(hipGraphLaunch(b->graph_exec, s));
+    return batch_order_end(b, s);
 }
 
 int stntt_batch_run_plain(stntt_batch *b, stntt_stream stream) {
@@ -1460,6 +1495,7 @@ int stntt_batch_destroy(stntt_batch *b) {
     if (!b) return STNTT_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->run_last) (void)hipEventSynchronize(b->run_ev);  // the last run, on any stream
     batch_free(b);
     delete b;
     return STNTT_OK;

@@ -83,3 +83,45 @@ def test_multipass_transforms_on_two_streams():
         for st in (s1, s2, s3):
             st.close()
         ctx.close()
+
+
+def test_batch_runs_on_two_streams_in_order():
+    """A batch's runs share its queue counter and group scratch: a run on a
+    second stream starts after the first (here also the data dependency:
+    the transforms are in place, so two runs apply each transform twice)."""
+    ctx = S.Context(0)
+    s1, s2 = HipStream(), HipStream()
+    try:
+        rng = np.random.default_rng(12)
+        specs = [("goldilocks", 10), ("u32", 12), ("goldilocks", 16), ("u32", 17), ("goldilocks", 13)]
+        xs, bufs, descs = [], [], []
+        for field, L in specs:
+            n = 1 << L
+            if field == "goldilocks":
+                x, root, mod = rng.integers(0, P, size=n, dtype=np.uint64), 0, P
+            else:
+                root, mod = S.find_params(n, 11)
+                x = rng.integers(0, mod, size=n, dtype=np.uint32)
+            xs.append((x, root, mod))
+            buf = ctx.upload(x)
+            bufs.append(buf)
+            descs.append((field, False, L, buf, root, mod))
+        ctx.sync()
+        b = S.Batch(ctx, descs)
+        b.run(stream=s1.ptr)
+        b.run(stream=s2.ptr)
+        s2.sync()
+        for (field, L), (x, root, mod), buf in zip(specs, xs, bufs):
+            n = 1 << L
+            if field == "goldilocks":
+                want = O.stockham_gl(O.stockham_gl(x, L), L)
+            else:
+                want = O.stockham_u32(O.stockham_u32(x, n, root, mod), n, root, mod)
+            assert np.array_equal(buf.copy_to(np.empty_like(x)), want), (field, L)
+        b.close()
+        for buf in bufs:
+            buf.free()
+    finally:
+        s1.close()
+        s2.close()
+        ctx.close()
