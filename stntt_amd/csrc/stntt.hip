@@ -1515,21 +1515,41 @@ int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t ns
     return STNTT_OK;
 }
 
-int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out) {
-    if (!out) return fail(STNTT_EINVAL, "null out pointer");
-    if (workers < 1 || workers > 16) return fail(STNTT_EINVAL, "workers %u outside [1, 16]", workers);
-    const uint64_t reserve = workers * server_worker_reserve(SERVER_MAX_LOG);
-    if (hbm_budget_bytes <= reserve)
-        return fail(STNTT_EINVAL, "budget %llu does not cover the worker reservation %llu",
-                    (unsigned long long)hbm_budget_bytes, (unsigned long long)reserve);
-    auto S = std::make_unique<stntt_server>();
-    S->device = device;
-    S->budget = hbm_budget_bytes - reserve;
+// Frees everything a server holds once its thread has stopped (or was never
+// started): streams, worker contexts and buffers, cached device buffers,
+// events.  Used by destroy and by create's failure paths.
+static void server_release(stntt_server *srv) {
+    (void)hipSetDevice(srv->device);
+    if (srv->up) (void)hipStreamSynchronize(srv->up);
+    if (srv->down) (void)hipStreamSynchronize(srv->down);
+    for (auto &W : srv->workers) {
+        if (W.stream) (void)hipStreamSynchronize(W.stream);
+        if (W.h_tasks) (void)hipHostFree(W.h_tasks);
+        if (W.d_tasks) (void)hipFree(W.d_tasks);
+        if (W.d_counter) (void)hipFree(W.d_counter);
+        if (W.d_scratch) (void)hipFree(W.d_scratch);
+        if (W.ctx) stntt_ctx_destroy(W.ctx);
+    }
+    srv->workers.clear();
+    for (auto &kv : srv->cache)
+        for (void *p : kv.second) (void)hipFree(p);
+    srv->cache.clear();
+    for (auto &kv : srv->reqs)
+        if (kv.second.d_buf) (void)hipFree(kv.second.d_buf);
+    for (hipEvent_t e : srv->free_events) (void)hipEventDestroy(e);
+    srv->free_events.clear();
+    if (srv->up) (void)hipStreamDestroy(srv->up);
+    if (srv->down) (void)hipStreamDestroy(srv->down);
+    srv->up = srv->down = nullptr;
+}
+
+static int server_setup(stntt_server *S, int device, uint32_t workers) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&S->up, hipStreamNonBlocking));
     HIP_TRY(hipStreamCreateWithFlags(&S->down, hipStreamNonBlocking));
     for (uint32_t i = 0; i < workers; ++i) {
-        ServerWorker W;
+        S->workers.emplace_back();
+        ServerWorker &W = S->workers.back();
         int rc = stntt_ctx_create(device, &W.ctx);
         if (rc) return rc;
         W.stream = W.ctx->stream;
@@ -1540,7 +1560,26 @@ int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers,
         HIP_TRY(hipMalloc(&W.d_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
         HIP_TRY(hipMalloc(&W.d_counter, sizeof(uint32_t)));
         HIP_TRY(hipMalloc(&W.d_scratch, (size_t)SERVER_BATCH_GRID * (8ull << SERVER_BATCH_MAX_LOG)));
-        S->workers.push_back(W);
+    }
+    return STNTT_OK;
+}
+
+int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out) {
+    if (!out) return fail(STNTT_EINVAL, "null out pointer");
+    if (workers < 1 || workers > 16) return fail(STNTT_EINVAL, "workers %u outside [1, 16]", workers);
+    const uint64_t reserve = workers * server_worker_reserve(SERVER_MAX_LOG);
+    if (hbm_budget_bytes <= reserve)
+        return fail(STNTT_EINVAL, "budget %llu does not cover the worker reservation %llu",
+                    (unsigned long long)hbm_budget_bytes, (unsigned long long)reserve);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return fail(STNTT_EINVAL, "device %d out of range", device);
+    auto S = std::make_unique<stntt_server>();
+    S->device = device;
+    S->budget = hbm_budget_bytes - reserve;
+    if (int rc = server_setup(S.get(), device, workers)) {
+        server_release(S.get());
+        return rc;
     }
     stntt_server *raw = S.get();
     S->thread = std::thread(server_loop, raw);
@@ -1556,24 +1595,7 @@ int stntt_server_destroy(stntt_server *srv) {
     }
     srv->cv_work.notify_all();
     if (srv->thread.joinable()) srv->thread.join();
-    (void)hipSetDevice(srv->device);
-    if (srv->up) (void)hipStreamSynchronize(srv->up);
-    if (srv->down) (void)hipStreamSynchronize(srv->down);
-    for (auto &W : srv->workers) {
-        (void)hipStreamSynchronize(W.stream);
-        (void)hipHostFree(W.h_tasks);
-        (void)hipFree(W.d_tasks);
-        (void)hipFree(W.d_counter);
-        (void)hipFree(W.d_scratch);
-        stntt_ctx_destroy(W.ctx);
-    }
-    for (auto &kv : srv->cache)
-        for (void *p : kv.second) (void)hipFree(p);
-    for (auto &kv : srv->reqs)
-        if (kv.second.d_buf) (void)hipFree(kv.second.d_buf);
-    for (hipEvent_t e : srv->free_events) (void)hipEventDestroy(e);
-    if (srv->up) (void)hipStreamDestroy(srv->up);
-    if (srv->down) (void)hipStreamDestroy(srv->down);
+    server_release(srv);
     delete srv;
     return STNTT_OK;
 }
@@ -1625,9 +1647,17 @@ int stntt_server_submit(stntt_server *srv, const stntt_request *req, uint64_t *i
 int stntt_server_wait(stntt_server *srv, uint64_t id, double *latency_ms) {
     if (!srv) return fail(STNTT_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(srv->mu);
-    auto it = srv->reqs.find(id);
-    if (it == srv->reqs.end()) return fail(STNTT_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
-    srv->cv_done.wait(lk, [&] { return it->second.done; });
+    if (srv->reqs.find(id) == srv->reqs.end())
+        return fail(STNTT_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    // look the request up again after every wake-up: another thread waiting
+    // on the same id may have collected (erased) it meanwhile
+    auto it = srv->reqs.end();
+    srv->cv_done.wait(lk, [&] {
+        it = srv->reqs.find(id);
+        return it == srv->reqs.end() || it->second.done;
+    });
+    if (it == srv->reqs.end())
+        return fail(STNTT_ENOTFOUND, "request %llu was collected by another waiter", (unsigned long long)id);
     int status = it->second.status;
     if (latency_ms) *latency_ms = it->second.latency_ms;
     srv->reqs.erase(it);

@@ -145,3 +145,31 @@ def test_server_close_frees_pinned_arrays():
     srv.close()
     assert data.ptr is None
     data.free()  # no-op
+
+
+def test_two_waiters_on_one_request():
+    """Exactly one of two threads waiting on the same request collects it;
+    the other is told it is gone (no use of a collected entry)."""
+    srv = S.Server(0, hbm_budget_bytes=RESERVE + (1 << 30), workers=1)
+    rng = np.random.default_rng(21)
+    data, root, mod = make_request(srv, rng, "goldilocks", 20, False)
+    x = data.array.copy()
+    rid = srv.submit("goldilocks", 20, data)
+    outcomes = []
+
+    def waiter():
+        try:
+            srv.wait(rid)
+            outcomes.append("ok")
+        except _lib.StnttError as e:
+            outcomes.append(e.code)
+
+    threads = [threading.Thread(target=waiter) for _ in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert sorted(outcomes, key=str) == sorted(["ok", _lib.STNTT_ENOTFOUND], key=str)
+    assert np.array_equal(data.array, expected("goldilocks", 20, False, x, root, mod))
+    data.free()
+    srv.close()
