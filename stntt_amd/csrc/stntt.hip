@@ -1096,13 +1096,20 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
     if (device < 0 || device >= count) return fail(STNTT_EINVAL, "device %d out of range", device);
     auto ctx = std::make_unique<stntt_ctx>();
     ctx->device = device;
-    HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
-    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&ctx->ev_start));
-    HIP_TRY(hipEventCreate(&ctx->ev_stop));
-    for (auto &e : ctx->prof_ev) HIP_TRY(hipEventCreate(&e));
-    HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming));
+    auto setup = [&]() -> int {
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ctx->ev_start));
+        HIP_TRY(hipEventCreate(&ctx->ev_stop));
+        for (auto &e : ctx->prof_ev) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming));
+        return STNTT_OK;
+    };
+    if (int rc = setup()) {
+        stntt_ctx_destroy(ctx.release());  // releases whatever was created
+        return rc;
+    }
     *out = ctx.release();
     return STNTT_OK;
 }
@@ -1115,11 +1122,12 @@ int stntt_ctx_destroy(stntt_ctx *ctx) {
     for (auto &kv : ctx->plans) (void)hipFree(kv.second->d_tables);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->staging) (void)hipFree(ctx->staging);
-    (void)hipEventDestroy(ctx->ev_start);
-    (void)hipEventDestroy(ctx->ev_stop);
-    for (auto &e : ctx->prof_ev) (void)hipEventDestroy(e);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    for (auto &e : ctx->prof_ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->scratch_ev) (void)hipEventDestroy(ctx->scratch_ev);
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return STNTT_OK;
 }
