@@ -62,28 +62,15 @@ GL_HD uint64_t gl_sub(uint64_t a, uint64_t b) {
     return gl_sub_borrow(a, b, d) ? d + GL_P : d;
 }
 
-// lo + hi * 2^64 (mod P) for any lo, hi; result is any 64-bit word
-// (callers canonicalise).  hi = h0 + h1 2^32:  lo + h0 (2^32 - 1) - h1.
-//   r = h0 * EPS + lo         one v_mad_u64_u32; a carry is worth EPS and
-//                             the corrected sum stays below 2^64
-//   r - h1                    a borrow is worth -EPS; r - h1 + 2^64 > 2^32,
-//                             so no second borrow
-GL_HD uint64_t gl_reduce128_w(uint64_t lo, uint64_t hi) {
-    uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
-    uint64_t r = h0 * GL_EPS + lo;
-    if (r < lo) r += GL_EPS;
-    uint64_t t;
-    if (gl_sub_borrow(r, h1, t)) t -= GL_EPS;
-    return t;
-}
-
-// The same with a canonical result, ordered for fewer instructions:
+// lo + hi * 2^64 (mod P) for any lo, hi; result in [0, P).
+// hi = h0 + h1 2^32 and 2^64 == EPS, 2^96 == -1:  lo + h0 EPS - h1.
 //   s = lo - h1                 a borrow is worth -EPS (s + P, no second
 //                               borrow: lo - h1 + 2^64 > EPS)
 //   r = h0 * EPS + s            one v_mad_u64_u32; s is any 64-bit word
 // A carry of r is worth +EPS, and then r + EPS <= 2^64 - 2^32 - 1 < P is
 // already canonical; without one, r - P = r + EPS (mod 2^64) is due exactly
-// when r + EPS wraps.  Both corrections are the one word r + EPS.
+// when r + EPS wraps.  Both corrections are the one word r + EPS (the other
+// order -- multiply-add, subtract, canonicalise -- took four more VALU).
 GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
     const uint64_t h0 = (uint32_t)hi, h1 = hi >> 32;
     uint64_t s;
@@ -110,32 +97,17 @@ GL_HD uint64_t gl_mul(uint64_t a, uint64_t b) {
     return gl_reduce128(lo, hi);
 }
 
-// x * 2^s (mod P) for 0 < s <= 32 and any x; result is any 64-bit word.
-// x << s = lo + h 2^64 with h < 2^s, so lo + h EPS by one multiply-add and
-// one carry correction (the corrected value stays below 2^64).
-GL_HD uint64_t gl_shl_small_w(uint64_t x, int s) {
-    uint64_t lo = x << s;
-    uint64_t h = x >> (64 - s);
-    uint64_t r = h * GL_EPS + lo;
-    if (r < lo) r += GL_EPS;
-    return r;
-}
-
-// The same with a canonical result in one correction: after a carry
-// r + EPS < 2^32 EPS < P is already canonical, and without one r - P =
-// r + EPS (mod 2^64) is due exactly when r >= P, i.e. when r + EPS wraps.
+// x * 2^s (mod P) for 0 < s <= 32 and any x; result in [0, P).
+// x << s = lo + h 2^64 with h < 2^s, so lo + h EPS by one multiply-add; after
+// a carry r + EPS < 2^32 EPS < P is already canonical, and without one
+// r - P = r + EPS (mod 2^64) is due exactly when r >= P, i.e. when r + EPS
+// wraps: one correction either way.
 GL_HD uint64_t gl_shl_small(uint64_t x, int s) {
     uint64_t lo = x << s;
     uint64_t h = x >> (64 - s);
     uint64_t r = h * GL_EPS + lo;
     uint64_t t = r + GL_EPS;
     return (r < lo || t < r) ? t : r;
-}
-
-// x * 2^s (mod P) for 0 < s < 64, any x; result is any 64-bit word
-GL_HD uint64_t gl_shl_w(uint64_t x, int s) {
-    if (s <= 32) return gl_shl_small_w(x, s);
-    return gl_reduce128_w(x << s, x >> (64 - s));
 }
 
 // x * 2^S (mod P) for a compile-time S and any x; result in [0, P).
