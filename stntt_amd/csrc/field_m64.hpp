@@ -39,25 +39,36 @@ M64_HD uint64_t m64_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t pinv) {
     gl_mul_wide(a, b, tlo, thi);
     const uint64_t m = tlo * pinv;
     gl_mul_wide(m, p, mlo, mhi);
-    (void)mlo;                         // tlo + mlo == 0 mod 2^64
-    const uint64_t c = tlo != 0;       // carry of tlo + mlo
-    uint64_t u = thi + mhi;
-    bool ov = u < thi;
-    const uint64_t u2 = u + c;
-    ov = ov || u2 < u;
-    const uint64_t r = u2 - p;
-    return (ov || u2 >= p) ? r : u2;
+    (void)mlo;  // tlo + mlo == 0 mod 2^64, with a carry exactly when tlo != 0
+    // u = thi + mhi + carry < 2p as one carry chain (the carry enters as its
+    // carry-in); one subtraction of p when u carried out or u - p does not
+    // borrow
+    unsigned c1, c2;
+    const uint32_t u0 = __builtin_addc((uint32_t)thi, (uint32_t)mhi, tlo != 0 ? 1u : 0u, &c1);
+    const uint32_t u1 = __builtin_addc((uint32_t)(thi >> 32), (uint32_t)(mhi >> 32), c1, &c2);
+    const uint64_t u = ((uint64_t)u1 << 32) | u0;
+    uint64_t r;
+    const bool below = gl_sub_borrow(u, p, r);  // u < p
+    return (c2 || !below) ? r : u;
 }
 
+// a + b (mod p) for a, b < p: a + b < 2p, so at most one subtraction, due
+// when the sum carried out of 64 bits or s - p does not borrow.  Both flags
+// come from two-word carry chains.
 M64_HD uint64_t m64_add(uint64_t a, uint64_t b, uint64_t p) {
-    const uint64_t s = a + b;  // a + b < 2p: at most one subtraction
-    const uint64_t r = s - p;
-    return (s < a || s >= p) ? r : s;
+    unsigned c1, c2;
+    const uint32_t s0 = __builtin_addc((uint32_t)a, (uint32_t)b, 0u, &c1);
+    const uint32_t s1 = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(b >> 32), c1, &c2);
+    const uint64_t s = ((uint64_t)s1 << 32) | s0;
+    uint64_t r;
+    const bool below = gl_sub_borrow(s, p, r);  // s < p
+    return (c2 || !below) ? r : s;
 }
 
+// a - b (mod p) for a, b < p; the subtraction's own borrow selects + p
 M64_HD uint64_t m64_sub(uint64_t a, uint64_t b, uint64_t p) {
-    const uint64_t d = a - b;
-    return a < b ? d + p : d;
+    uint64_t d;
+    return gl_sub_borrow(a, b, d) ? d + p : d;
 }
 
 // host-side constants of a modulus
