@@ -97,6 +97,16 @@ __device__ __forceinline__ global_t<T> &at_off(T *base, uint32_t elem) {
     return *(global_t<T> *)((char *)base + (uint64_t)elem * sizeof(T));
 }
 
+// A word of a twiddle table through the constant address space (4): the
+// tables are never written while a transform runs, so a load at a uniform
+// address becomes a scalar load into SGPRs instead of a vector load that
+// holds a VGPR in every lane (the in-register DFT roots: 24 VGPRs for a
+// 32-bit radix-512 tile, 48 for the 64-bit field).
+template <class T>
+__device__ __forceinline__ T ld_table(const T *base, uint32_t elem) {
+    return *(const __attribute__((address_space(4))) T *)((const char *)base + (uint64_t)elem * sizeof(T));
+}
+
 // bit reversal of the low `bits` bits of x, 1 <= bits <= 32
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits) { return __brev(x) >> (32 - bits); }
 
@@ -166,7 +176,7 @@ struct FieldU32 {
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
         Roots r;
 #pragma unroll
-        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = at_off(hi, (uint32_t)(j << (log_rmax - LR)));
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = ld_table(hi, (uint32_t)(j << (log_rmax - LR)));
         return r;
     }
     template <int LR, int DIR, int J>
@@ -208,7 +218,7 @@ struct FieldM64 {
     static __device__ __forceinline__ Roots load_roots(const Tw *hi, int log_rmax) {
         Roots r;
 #pragma unroll
-        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = at_off(hi, (uint32_t)(j << (log_rmax - LR)));
+        for (int j = 0; j < (1 << LR) / 2; ++j) r.w[j] = ld_table(hi, (uint32_t)(j << (log_rmax - LR)));
         return r;
     }
     template <int LR, int DIR, int J>
