@@ -52,6 +52,26 @@ def test_u64_matches_oracle(ctx, L, batches, min_mod, inverse):
     assert np.array_equal(buf.copy_to(np.empty_like(x)), want)
 
 
+@pytest.mark.parametrize("split", [[9, 5, 6], [5, 9, 6], [6, 5, 9]])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_u64_radix512_passes(ctx, split, inverse):
+    """Radix-512 tiles (exchange halves, the per-tile LDS root table) as the
+    first, a middle and the last pass, above 2^63."""
+    L = sum(split)
+    n = 1 << L
+    root, mod = S.find_params(n, 2**63 + 1)
+    x = words(n, 77 + split[0])
+    want = O.inverse_u64(x, n, root, mod) if inverse else O.stockham_u64(x, n, root, mod)
+    S.set_pass_split(ctx, "u64", L, split)
+    try:
+        buf = ctx.upload(x)
+        ctx.ntt_u64(buf, L, 1, root, mod, inverse=inverse)
+        assert np.array_equal(buf.copy_to(np.empty_like(x)), want), split
+        buf.free()
+    finally:
+        S.set_pass_split(ctx, "u64", L, None)
+
+
 @pytest.mark.parametrize("L", [12, 20, 24])
 def test_u64_at_goldilocks_equals_goldilocks_kernels(ctx, L):
     """Montgomery arithmetic at p = P against the special-form Goldilocks
