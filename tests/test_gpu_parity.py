@@ -280,7 +280,9 @@ def test_errors(ctx):
 
 @pytest.mark.parametrize("field,L,split", [("goldilocks", 24, [9, 8, 7]), ("goldilocks", 16, [6, 5, 5]),
                                            ("u32", 20, [6, 7, 7]), ("u32", 22, [9, 9, 4]),
-                                           ("goldilocks", 20, [5, 5, 5, 5])])
+                                           ("goldilocks", 20, [5, 5, 5, 5]),
+                                           # radix-512 last passes (row halves, final scaling)
+                                           ("goldilocks", 20, [6, 5, 9]), ("u32", 21, [6, 6, 9])])
 def test_tuned_pass_splits_same_result(ctx, field, L, split):
     """Any valid stages-per-pass split gives the same bits."""
     n = 1 << L
@@ -294,6 +296,7 @@ def test_tuned_pass_splits_same_result(ctx, field, L, split):
             root, mod = S.find_params(n, 11)
             x = u32_input(3000 + L, n, mod)
             assert np.array_equal(run_u32(ctx, x, L, 1, root, mod), O.stockham_u32(x, n, root, mod))
+            assert np.array_equal(run_u32(ctx, x, L, 1, root, mod, inverse=True), O.inverse_u32(x, n, root, mod))
     finally:
         S.set_pass_split(ctx, field, L, None)
 
