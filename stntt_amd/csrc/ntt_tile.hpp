@@ -144,11 +144,14 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     const auto roots = FP::template load_roots<LR1, DIR>(a.hi, a.log_rmax);
     const uint32_t lt = logn - LOGR;  // bits of a column index t
     if constexpr (PERM == PERM_IN) {
+        // static_for rather than an unroll pragma: with the 64-bit Montgomery
+        // product the unrolled loop exceeds the compiler's size limit, the
+        // loop stays rolled and sv/su move to scratch memory
         T sv[SP], su[SP];
         const bool pw = a.mul_in != nullptr;
         const T *mul = pw ? a.mul_in + base : nullptr;
-#pragma unroll
-        for (int i = 0; i < SP; ++i) {
+        static_for<SP>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
             const uint32_t idx = tid + i * NT;
             if ((R * C) % NT == 0 || idx < (uint32_t)(R * C)) {
                 const uint32_t p = idx & (R - 1), c = idx >> LOGR;
@@ -156,10 +159,10 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 sv[i] = at_off(in, off);
                 if (pw) su[i] = at_off(mul, off);
             }
-        }
+        });
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < SP; ++i) {
+        static_for<SP>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
             const uint32_t idx = tid + i * NT;
             if ((R * C) % NT == 0 || idx < (uint32_t)(R * C)) {
                 const uint32_t p = idx & (R - 1), c = idx >> LOGR;
@@ -168,7 +171,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 const uint32_t m = brev_bits(p, LOGR);
                 lds[m * C + (c ^ (m & (C - 1)))] = x;
             }
-        }
+        });
         __syncthreads();
     }
     [[maybe_unused]] T xo[HALF ? r1 : 1];  // HALF: sub-step 1 outputs until the thread's round
@@ -329,9 +332,13 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         xc[gi] = c;
         xg2[gi] = g2;
     };
+    // the rounds and groups below expand at compile time (static_for): a
+    // rolled loop would index xc / xg2 at run time (scratch memory) and, for
+    // the 64-bit Montgomery field, the unroll pragma alone exceeds the
+    // compiler's size limit
     if constexpr (HALF_COLS) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        static_for<2>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
             if ((xo_c >> (LOGC - 1)) == (uint32_t)h) {  // this thread's column is in half h
 #pragma unroll
                 for (int k = 0; k < r1; ++k) {
@@ -343,13 +350,13 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             // lanes walk g2: a column's outputs are contiguous
             const uint32_t g2 = tid & (r1 - 1), lc = tid >> LR1;
             substep2(h * CL + lc, g2, lc, g2, h);
-            if (h == 0) __syncthreads();  // round 0's reads before round 1's writes
-        }
+            if constexpr (h == 0) __syncthreads();  // round 0's reads before round 1's writes
+        });
     } else if constexpr (HALF_ROWS) {
         // round h: groups g2 in [h r1/2, (h+1) r1/2), i.e. sub-step 1 outputs
         // k in that range; output k of group g sits at image row g r1/2 + k'
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        static_for<2>([&](auto hc) {
+            constexpr int h = decltype(hc)::value;
 #pragma unroll
             for (int k = 0; k < r1 / 2; ++k) {
                 const uint32_t row = (r1 / 2) * xo_g + k;
@@ -359,19 +366,19 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             // lanes walk c: a row of C outputs is contiguous
             const uint32_t c = tid & (C - 1), g2l = tid >> LOGC;
             substep2(c, h * (r1 / 2) + g2l, c, g2l, h);
-            if (h == 0) __syncthreads();  // round 0's reads before round 1's writes
-        }
+            if constexpr (h == 0) __syncthreads();  // round 0's reads before round 1's writes
+        });
     } else {
-#pragma unroll
-        for (int gi = 0; gi < G2; ++gi) {
+        static_for<G2>([&](auto gc) {
+            constexpr int gi = decltype(gc)::value;
             const uint32_t q = tid + gi * NT;
-            if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) continue;
+            if (!(NG2 % NT == 0 || q < (uint32_t)NG2)) return;
             if constexpr (J1) {  // outputs of a column are contiguous: lanes walk g2
                 substep2(q >> LR1, q & (r1 - 1), q >> LR1, q & (r1 - 1), gi);
             } else {  // outputs of a row are contiguous: lanes walk c
                 substep2(q & (C - 1), q >> LOGC, q & (C - 1), q >> LOGC, gi);
             }
-        }
+        });
     }
     if constexpr (PERM == PERM_OUT) {
         __syncthreads();  // every sub-step 2 read of the tile is done

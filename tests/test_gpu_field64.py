@@ -72,6 +72,36 @@ def test_u64_radix512_passes(ctx, split, inverse):
         S.set_pass_split(ctx, "u64", L, None)
 
 
+@pytest.mark.parametrize("split", [[9, 5, 6], [6, 5, 9]])
+def test_u64_radix512_orders_and_polymul(ctx, split):
+    """Radix-512 first / last passes that stage the tile for bit-reversed
+    input (with and without the fused pointwise operand) and output."""
+    L = sum(split)
+    n = 1 << L
+    root, mod = S.find_params(n, 2**63 + 1)
+    print(f"u64 radix-512 orders/polymul split={split}", flush=True)
+    x = words(n, 300 + split[0]) % np.uint64(mod)
+    perm = bitrev_perm(L)
+    want = O.stockham_u64(x, n, root, mod)
+    S.set_pass_split(ctx, "u64", L, split)
+    try:
+        buf = ctx.upload(x[perm].copy())
+        ctx.ntt_u64(buf, L, 1, root, mod, bitrev_in=True)
+        assert np.array_equal(buf.copy_to(np.empty_like(x)), want), "bit-reversed input"
+        buf.copy_from(x)
+        ctx.ntt_u64(buf, L, 1, root, mod, bitrev_out=True)
+        assert np.array_equal(buf.copy_to(np.empty_like(x)), want[perm]), "bit-reversed output"
+        buf.free()
+        a, b = words(n, 11) % np.uint64(mod), words(n, 12) % np.uint64(mod)
+        da, db = ctx.upload(a), ctx.upload(b)
+        ctx.polymul_u64(da, db, L, 1, root, mod)
+        assert np.array_equal(da.copy_to(np.empty_like(a)), O.pwc_ntt_u64(a, b, root, mod)), "polymul"
+        da.free()
+        db.free()
+    finally:
+        S.set_pass_split(ctx, "u64", L, None)
+
+
 @pytest.mark.parametrize("L", [12, 20, 24])
 def test_u64_at_goldilocks_equals_goldilocks_kernels(ctx, L):
     """Montgomery arithmetic at p = P against the special-form Goldilocks
