@@ -43,8 +43,11 @@ P = 0xFFFFFFFF00000001
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # about 70 ms of warm-up: the GPU clock ramps up over the first ~50 ms
+    # of load (a 2^24 Goldilocks transform takes ~0.26 ms cold, ~0.22 ms
+    # after the ramp), then about 0.2 s of timed transforms
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--log2n", type=int, default=24)
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"),
                     help="FETCH_SIZE/WRITE_SIZE per dispatch measured by rocprofv3 (optional)")
@@ -101,14 +104,7 @@ def bench_n20(S, ctx, steps: int) -> dict:
             else:
                 ctx.ntt_goldilocks(buf, 20, 1)
                 ctx.ntt_goldilocks(buf, 20, 1, inverse=True)
-        for _ in range(5):
-            step()
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step()
-        ctx.sync()
-        dt = (time.perf_counter() - t0) / steps
+        dt, _ = _time_steps(ctx, step, steps)  # every step is a round trip
         ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
         out[field] = {"us_per_fwd_inv": round(dt * 1e6, 2), "roundtrip_exact": bool(ok),
                       "gelem_s": round(2 * n / dt / 1e9, 3)}
@@ -116,15 +112,22 @@ def bench_n20(S, ctx, steps: int) -> dict:
     return out
 
 
-def _time_steps(ctx, step, steps: int) -> float:
-    for _ in range(3):
+def _time_steps(ctx, step, steps: int, settle_s: float = 0.08) -> tuple[float, int]:
+    """Seconds per step over `steps` back-to-back steps, after at least 3
+    untimed steps and `settle_s` of the same work (the input upload before a
+    leg leaves the GPU idle, and its clock ramps up again over ~50 ms of
+    load).  Returns (seconds per step, steps run in total)."""
+    calls = 0
+    t_end = time.perf_counter() + settle_s
+    while calls < 3 or time.perf_counter() < t_end:
         step()
-    ctx.sync()
+        ctx.sync()
+        calls += 1
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     ctx.sync()
-    return (time.perf_counter() - t0) / steps
+    return (time.perf_counter() - t0) / steps, calls + steps
 
 
 def bench_orders_polymul(S, ctx, steps: int) -> dict:
@@ -143,7 +146,7 @@ def bench_orders_polymul(S, ctx, steps: int) -> dict:
     orders = {}
     for name, kw in (("natural", {}), ("bitrev_out", {"bitrev_out": True}),
                      ("bitrev_in", {"bitrev_in": True})):
-        orders[name] = round(_time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1, **kw), steps) * 1e3, 4)
+        orders[name] = round(_time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1, **kw), steps)[0] * 1e3, 4)
     out["goldilocks_2^24_forward_ms"] = orders
     buf.free()
 
@@ -186,10 +189,10 @@ def bench_orders_polymul(S, ctx, steps: int) -> dict:
         def copies():
             reset()
 
-        t_copy = _time_steps(ctx, copies, steps)
-        t_fused = _time_steps(ctx, fused, steps) - t_copy
+        t_copy = _time_steps(ctx, copies, steps)[0]
+        t_fused = _time_steps(ctx, fused, steps)[0] - t_copy
         got_fused = ra.copy_to(np.empty_like(a))
-        t_unfused = _time_steps(ctx, unfused, steps) - t_copy
+        t_unfused = _time_steps(ctx, unfused, steps)[0] - t_copy
         got_unfused = ra.copy_to(np.empty_like(a))
         alg = 7 * batches * n * w
         out[f"polymul_{field}_2^{L}_x{batches}"] = {
@@ -212,9 +215,15 @@ def bench_u32_large(S, ctx, steps: int) -> dict:
         root, mod = S.find_params(n, 11)
         x = np.random.default_rng(L).integers(0, mod, size=n, dtype=np.uint32)
         buf = ctx.upload(x)
-        fw = _time_steps(ctx, lambda: ctx.ntt_u32(buf, L, 1, root, mod), steps)
-        iv = _time_steps(ctx, lambda: ctx.ntt_u32(buf, L, 1, root, mod, inverse=True), steps)
-        # as many inverses as forwards ran (steps + 3 each): x is back
+        fwd = lambda: ctx.ntt_u32(buf, L, 1, root, mod)  # noqa: E731
+        inv = lambda: ctx.ntt_u32(buf, L, 1, root, mod, inverse=True)  # noqa: E731
+        fw, nf = _time_steps(ctx, fwd, steps)
+        iv, ni = _time_steps(ctx, inv, steps)
+        # as many inverses as forwards: x is back
+        for _ in range(nf - ni):
+            inv()
+        for _ in range(ni - nf):
+            fwd()
         ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
         out[f"2^{L}"] = {"mod": mod, "forward_us": round(fw * 1e6, 2), "inverse_us": round(iv * 1e6, 2),
                          "gelem_s": round(n / fw / 1e9, 2), "roundtrip_exact": bool(ok)}
@@ -233,9 +242,9 @@ def bench_field64(S, ctx, steps: int) -> dict:
     buf = ctx.upload(x)
     for name, min_mod in (("p_above_2^63", 2**63 + 1), ("p_goldilocks", P)):
         root, mod = S.find_params(n, min_mod)
-        t = _time_steps(ctx, lambda: ctx.ntt_u64(buf, L, 1, root, mod), steps)
+        t = _time_steps(ctx, lambda: ctx.ntt_u64(buf, L, 1, root, mod), steps)[0]
         out[name] = {"mod": mod, "forward_ms": round(t * 1e3, 4), "gelem_s": round(n / t / 1e9, 3)}
-    t = _time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1), steps)
+    t = _time_steps(ctx, lambda: ctx.ntt_goldilocks(buf, L, 1), steps)[0]
     out["goldilocks_kernels_forward_ms"] = round(t * 1e3, 4)
     buf.free()
     return out
@@ -269,23 +278,10 @@ def bench_batched(S, ctx, count: int, reps: int, plain: bool = True) -> dict:
     batch.run()
     ctx.sync()
     batch.capture()
-    batch.run_graph()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        batch.run_graph()
-    ctx.sync()
-    t_batch = (time.perf_counter() - t0) / reps
+    t_batch = _time_steps(ctx, batch.run_graph, reps)[0]
     t_plain = float("nan")
     if plain:
-        batch.run_plain()
-        ctx.sync()
-        t0 = time.perf_counter()
-        plain_reps = max(1, reps // 4)
-        for _ in range(plain_reps):
-            batch.run_plain()
-        ctx.sync()
-        t_plain = (time.perf_counter() - t0) / plain_reps
+        t_plain = _time_steps(ctx, batch.run_plain, max(1, reps // 4))[0]
     batch.close()
     for b in bufs:
         b.free()
@@ -411,7 +407,7 @@ def main() -> int:
     # per-kernel durations with HIP events on the library's stream
     ctx.set_profiling(True)
     per_pass: list[list[float]] = []
-    for _ in range(max(args.steps, 10)):
+    for _ in range(min(max(args.steps, 10), 200)):
         ctx.ntt_goldilocks(buf, L, 1)
         per_pass.append(ctx.profile_read())
     ctx.set_profiling(False)

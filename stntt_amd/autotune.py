@@ -6,10 +6,12 @@ Two choices are tuned and installed into a context through the C ABI:
   * the radix split of the batched kernel's sub-steps (stntt_set_batch_split),
     timed on a batch of same-shape transforms in one persistent launch.
 The winners are written to tuning/gfx950.json next to this file and applied
-to every new Context (Context(tuned=False) skips it).
+to every new Context (Context(tuned=False) skips it).  The 64-bit prime field
+("u64") has pass splits only (timed at a prime above 2^63; the Montgomery
+kernels cost the same for every odd modulus).
 
     python -m stntt_amd.autotune            # on an MI355X: everything
-    python -m stntt_amd.autotune --fields u32 --parts pass
+    python -m stntt_amd.autotune --fields u32,u64 --parts pass
         # re-tune a subset; entries not re-tuned are kept from the table
 """
 from __future__ import annotations
@@ -25,7 +27,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning", "gfx950.json")
 GL_P = 0xFFFFFFFF00000001
-FIELDS = ("u32", "goldilocks")
+FIELDS = ("u32", "goldilocks")  # pass and batch splits
+PASS_FIELDS = FIELDS + ("u64",)  # pass splits only
+
+
+def word_log2c(field: str) -> int:
+    """log2 of the tile columns: 128-byte rows of 4- or 8-byte words."""
+    return 5 if field == "u32" else 4
 
 
 def apply_table(ctx, path: str = TABLE) -> int:
@@ -36,10 +44,11 @@ def apply_table(ctx, path: str = TABLE) -> int:
     with open(path) as f:
         table = json.load(f)
     k = 0
-    for field in FIELDS:
+    for field in PASS_FIELDS:
         for L, logr in table.get("pass_split", {}).get(field, {}).items():
             set_pass_split(ctx, field, int(L), logr)
             k += 1
+    for field in FIELDS:
         for L, logs in table.get("batch_split", {}).get(field, {}).items():
             set_batch_split(ctx, field, int(L), logs)
             k += 1
@@ -85,7 +94,6 @@ def batch_candidates(L: int, streamed: bool) -> list[list[int]]:
 
 
 def _time(fn, sync, reps: int) -> float:
-    fn()
     sync()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -94,45 +102,73 @@ def _time(fn, sync, reps: int) -> float:
     return (time.perf_counter() - t0) / reps
 
 
+def measure(cands, install, fn, sync, reps: int, rounds: int = 3, warm_s: float = 0.3) -> dict:
+    """Median time per candidate over `rounds` rounds in alternating order.
+
+    The GPU clock ramps up over the first tenths of a second of load (and
+    after every host-side pause, such as preparing the next input), so every
+    candidate first runs once (plans, scratch), the workload then runs for
+    `warm_s`, and no candidate is always timed first: timed in a fixed order
+    right after an upload, the first candidate read up to 20 % slow."""
+    for c in cands:
+        install(c)
+        fn()
+    sync()
+    t_end = time.perf_counter() + warm_s
+    while time.perf_counter() < t_end:
+        fn()
+        sync()
+    acc = {str(c): [] for c in cands}
+    for r in range(rounds):
+        for c in (cands if r % 2 == 0 else cands[::-1]):
+            install(c)
+            acc[str(c)].append(_time(fn, sync, reps))
+    return {k: float(np.median(v)) for k, v in acc.items()}
+
+
 def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True, fields=FIELDS,
          parts=("pass", "batch")) -> dict:
     import stntt_amd as S
     from .ntt import set_batch_split, set_pass_split
     ctx = S.Context(0, tuned=False)
     rng = np.random.default_rng(1)
-    table = {"device": "gfx950", "pass_split": {f: {} for f in FIELDS}, "batch_split": {f: {} for f in FIELDS},
-             "measured_us": {"pass": {f: {} for f in FIELDS}, "batch": {f: {} for f in FIELDS}}}
+    table = {"device": "gfx950", "pass_split": {f: {} for f in PASS_FIELDS}, "batch_split": {f: {} for f in FIELDS},
+             "measured_us": {"pass": {f: {} for f in PASS_FIELDS}, "batch": {f: {} for f in FIELDS}}}
     if "pass" not in parts:
         logs_pass = ()
     if "batch" not in parts:
         logs_batch = ()
     for field in fields:
-        logc = 4 if field == "goldilocks" else 5
+        logc = word_log2c(field)
         for L in logs_pass:
             n = 1 << L
             if field == "u32":
                 root, mod = S.find_params(n, 11)
                 x = rng.integers(0, mod, size=n, dtype=np.uint32)
+            elif field == "u64":
+                root, mod = S.find_params(n, 2**63 + 1)
+                x = rng.integers(0, mod, size=n, dtype=np.uint64)
             else:
                 x = rng.integers(0, GL_P, size=n, dtype=np.uint64)
             buf = ctx.upload(x)
-            best, best_t, times = None, 1e9, {}
-            for cand in pass_candidates(L, logc):
-                set_pass_split(ctx, field, L, cand)
-                if field == "u32":
-                    t = _time(lambda: ctx.ntt_u32(buf, L, 1, root, mod), ctx.sync, 20)
-                else:
-                    t = _time(lambda: ctx.ntt_goldilocks(buf, L, 1), ctx.sync, 20)
-                times[str(cand)] = round(t * 1e6, 2)
-                if t < best_t:
-                    best, best_t = cand, t
+            if field == "u32":
+                fn = lambda: ctx.ntt_u32(buf, L, 1, root, mod)  # noqa: E731
+            elif field == "u64":
+                fn = lambda: ctx.ntt_u64(buf, L, 1, root, mod)  # noqa: E731
+            else:
+                fn = lambda: ctx.ntt_goldilocks(buf, L, 1)  # noqa: E731
+            cands = pass_candidates(L, logc)
+            med = measure(cands, lambda c: set_pass_split(ctx, field, L, c), fn, ctx.sync, 20)
+            times = {k: round(v * 1e6, 2) for k, v in med.items()}
+            best = min(cands, key=lambda c: med[str(c)])
+            best_t = med[str(best)]
             set_pass_split(ctx, field, L, best)
             table["pass_split"][field][str(L)] = best
             table["measured_us"]["pass"][field][str(L)] = times
             if verbose:
                 print(f"[tune] pass {field} 2^{L}: {best} {best_t * 1e6:.1f} us", flush=True)
             buf.free()
-        for L in logs_batch:
+        for L in logs_batch if field in FIELDS else ():
             n = 1 << L
             count = max(64, (1 << 22) >> L)
             w = 8 if field == "goldilocks" else 4
@@ -146,15 +182,20 @@ def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True
                 x = rng.integers(0, mod if field == "u32" else GL_P, size=n,
                                  dtype=np.uint32 if field == "u32" else np.uint64)
                 bufs.append(ctx.upload(x))
-            best, best_t, times = None, 1e9, {}
-            for cand in batch_candidates(L, streamed):
+            # the split is fixed when a batch is built: one batch per candidate
+            cands = batch_candidates(L, streamed)
+            batches = {}
+            for cand in cands:
                 set_batch_split(ctx, field, L, cand)
-                b = S.Batch(ctx, [(field, bool(i & 1), L, bb, root, mod) for i, bb in enumerate(bufs)])
-                t = _time(b.run, ctx.sync, 5)
+                batches[str(cand)] = S.Batch(ctx, [(field, bool(i & 1), L, bb, root, mod)
+                                                   for i, bb in enumerate(bufs)])
+            cur = {}
+            med = measure(cands, lambda c: cur.update(b=batches[str(c)]), lambda: cur["b"].run(), ctx.sync, 5)
+            for b in batches.values():
                 b.close()
-                times[str(cand)] = round(t * 1e6, 2)
-                if t < best_t:
-                    best, best_t = cand, t
+            times = {k: round(v * 1e6, 2) for k, v in med.items()}
+            best = min(cands, key=lambda c: med[str(c)])
+            best_t = med[str(best)]
             set_batch_split(ctx, field, L, best)
             table["batch_split"][field][str(L)] = best
             table["measured_us"]["batch"][field][str(L)] = times
@@ -183,12 +224,13 @@ def main(argv=None) -> int:
     import argparse
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("out", nargs="?", default=TABLE)
-    ap.add_argument("--fields", default=",".join(FIELDS), help="comma list of u32, goldilocks")
+    ap.add_argument("--fields", default=",".join(PASS_FIELDS),
+                    help="comma list of u32, goldilocks, u64 (u64: pass splits only)")
     ap.add_argument("--parts", default="pass,batch", help="comma list of pass, batch")
     args = ap.parse_args(argv)
     fields = tuple(f for f in args.fields.split(",") if f)
     parts = tuple(p for p in args.parts.split(",") if p)
-    if any(f not in FIELDS for f in fields) or any(p not in ("pass", "batch") for p in parts):
+    if any(f not in PASS_FIELDS for f in fields) or any(p not in ("pass", "batch") for p in parts):
         ap.error("unknown field or part")
     table = tune(fields=fields, parts=parts)
     if os.path.exists(args.out):

@@ -33,7 +33,9 @@ def test_shipped_table_is_valid():
     with open(A.TABLE) as f:
         table = json.load(f)
     for field, entries in table["pass_split"].items():
-        logc = 4 if field == "goldilocks" else 5
+        logc = A.word_log2c(field)
         for L, split in entries.items():
             assert sum(split) == int(L) and all(sum(split[:i]) >= logc for i in range(1, len(split))), (field, L)
+    assert set(table["pass_split"]) <= set(A.PASS_FIELDS)
+    assert set(table["batch_split"]) <= set(A.FIELDS)
     assert os.path.basename(A.TABLE) == "gfx950.json"
