@@ -5,8 +5,10 @@ Two choices are tuned and installed into a context through the C ABI:
     timed on a single resident transform of that size;
   * the radix split of the batched kernel's sub-steps (stntt_set_batch_split),
     timed on a batch of same-shape transforms in one persistent launch.
-The winners are written to tuning/gfx950.json next to this file and applied
-to every new Context (Context(tuned=False) skips it).  The 64-bit prime field
+The winners are written to tuning/gfx950.json next to this file, applied to
+every new Context (Context(tuned=False) skips the file) and compiled into the
+library by the next build (tuning/gen_tuned.py -> csrc/tuned_gfx950.hpp), so
+contexts created through the C ABI start with them too.  The 64-bit prime field
 ("u64") has pass splits only (timed at a prime above 2^63; the Montgomery
 kernels cost the same for every odd modulus).
 

@@ -23,6 +23,8 @@ SOURCES = ["stntt.hip"]
 
 
 def build(verbose: bool = False) -> str:
+    from .tuning.gen_tuned import generate
+    generate()  # csrc/tuned_gfx950.hpp from tuning/gfx950.json
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
     cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]

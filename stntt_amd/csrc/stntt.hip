@@ -19,6 +19,7 @@
 #include "ntt_pass.hpp"
 #include "plan.hpp"
 #include "server.hpp"
+#include "tuned_gfx950.hpp"
 
 namespace stntt {
 
@@ -1104,6 +1105,12 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
         HIP_TRY(hipEventCreate(&ctx->ev_stop));
         for (auto &e : ctx->prof_ev) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming));
+        // the splits tuned on this architecture (tuned_gfx950.hpp); callers
+        // may replace any of them with stntt_set_pass_split / _batch_split
+        for (const TunedPassSplit &t : TUNED_PASS_SPLITS)
+            if (int rc = stntt_set_pass_split(ctx.get(), t.field, t.log2n, t.npasses, t.logr)) return rc;
+        for (const TunedBatchSplit &t : TUNED_BATCH_SPLITS)
+            if (int rc = stntt_set_batch_split(ctx.get(), t.field, t.log2n, t.nsub, t.radix_logs)) return rc;
         return STNTT_OK;
     };
     if (int rc = setup()) {

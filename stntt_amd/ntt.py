@@ -46,7 +46,12 @@ def _log2_exact(n: int) -> int:
 
 
 class Context:
-    """One device: its stream, resident twiddle plans and scratch pool."""
+    """One device: its stream, resident twiddle plans and scratch pool.
+
+    Every context starts with the splits compiled into the library
+    (csrc/tuned_gfx950.hpp, generated from tuning/gfx950.json at build
+    time); ``tuned=True`` re-applies tuning/gfx950.json itself, so a table
+    re-tuned since the last build takes effect without rebuilding."""
 
     def __init__(self, device: int = 0, tuned: bool = True):
         h = ctypes.c_void_p()

@@ -1,0 +1,79 @@
+"""Compile the tuning table into the library.
+
+Reads gfx950.json (written by ``python -m stntt_amd.autotune``) and writes
+``csrc/tuned_gfx950.hpp``, which every new context installs, so callers of
+the C ABI get the tuned splits without the Python package.  ``build()``
+runs this before compiling; a CPU test checks the committed header is in
+sync with the table.
+
+    python -m stntt_amd.tuning.gen_tuned
+"""
+from __future__ import annotations
+
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "gfx950.json")
+HEADER = os.path.join(os.path.dirname(HERE), "csrc", "tuned_gfx950.hpp")
+FIELD_IDS = {"u32": 0, "goldilocks": 1, "u64": 2}  # stntt_field
+
+
+def render(table: dict) -> str:
+    lines = [
+        "// Generated by stntt_amd/tuning/gen_tuned.py from stntt_amd/tuning/gfx950.json",
+        "// (python -m stntt_amd.autotune on an MI355X).  Do not edit by hand.",
+        "#pragma once",
+        "",
+        "#include <cstdint>",
+        "",
+        "namespace stntt {",
+        "",
+        "// stages per tile pass of the multi-pass path for one transform",
+        "struct TunedPassSplit {",
+        "    int field;",
+        "    uint32_t log2n;",
+        "    uint32_t npasses;",
+        "    uint32_t logr[4];",
+        "};",
+        "",
+        "// sub-step radices (log2, 4 bits each from the lowest) of the batched kernel",
+        "struct TunedBatchSplit {",
+        "    int field;",
+        "    uint32_t log2n;",
+        "    uint32_t nsub;",
+        "    uint32_t radix_logs;",
+        "};",
+        "",
+        "inline constexpr TunedPassSplit TUNED_PASS_SPLITS[] = {",
+    ]
+    for field, entries in sorted(table.get("pass_split", {}).items(), key=lambda kv: FIELD_IDS[kv[0]]):
+        for L, split in sorted(entries.items(), key=lambda kv: int(kv[0])):
+            assert 2 <= len(split) <= 4 and sum(split) == int(L), (field, L, split)
+            padded = list(split) + [0] * (4 - len(split))
+            lines.append(f"    {{{FIELD_IDS[field]}, {int(L)}, {len(split)}, {{{', '.join(map(str, padded))}}}}},"
+                         f"  // {field}")
+    lines += ["};", "", "inline constexpr TunedBatchSplit TUNED_BATCH_SPLITS[] = {"]
+    for field, entries in sorted(table.get("batch_split", {}).items(), key=lambda kv: FIELD_IDS[kv[0]]):
+        for L, logs in sorted(entries.items(), key=lambda kv: int(kv[0])):
+            assert 1 <= len(logs) <= 8 and sum(logs) == int(L) and all(1 <= x <= 15 for x in logs), (field, L, logs)
+            packed = 0
+            for i, lr in enumerate(logs):
+                packed |= lr << (4 * i)
+            lines.append(f"    {{{FIELD_IDS[field]}, {int(L)}, {len(logs)}, 0x{packed:x}u}},  // {field} {logs}")
+    lines += ["};", "", "}  // namespace stntt", ""]
+    return "\n".join(lines)
+
+
+def generate(table_path: str = TABLE, header_path: str = HEADER) -> str:
+    with open(table_path) as f:
+        text = render(json.load(f))
+    old = open(header_path).read() if os.path.exists(header_path) else None
+    if old != text:
+        with open(header_path, "w") as f:
+            f.write(text)
+    return header_path
+
+
+if __name__ == "__main__":
+    print(generate())

@@ -3,6 +3,7 @@ import json
 import os
 
 from stntt_amd import autotune as A
+from stntt_amd.tuning import gen_tuned as G
 
 
 def test_pass_candidates_cover_every_stage():
@@ -39,3 +40,18 @@ def test_shipped_table_is_valid():
     assert set(table["pass_split"]) <= set(A.PASS_FIELDS)
     assert set(table["batch_split"]) <= set(A.FIELDS)
     assert os.path.basename(A.TABLE) == "gfx950.json"
+
+
+def test_compiled_table_matches_json():
+    """csrc/tuned_gfx950.hpp (installed by every new context) is the
+    rendering of tuning/gfx950.json."""
+    with open(A.TABLE) as f:
+        table = json.load(f)
+    with open(G.HEADER) as f:
+        assert f.read() == G.render(table)
+    # every tuned split appears as a row of the header
+    text = G.render(table)
+    for field, entries in table["pass_split"].items():
+        for L, split in entries.items():
+            row = ", ".join(map(str, list(split) + [0] * (4 - len(split))))
+            assert f"{{{G.FIELD_IDS[field]}, {L}, {len(split)}, {{{row}}}}}" in text, (field, L)
