@@ -17,6 +17,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "gfx950.json")
 HEADER = os.path.join(os.path.dirname(HERE), "csrc", "tuned_gfx950.hpp")
 FIELD_IDS = {"u32": 0, "goldilocks": 1, "u64": 2}  # stntt_field
+MIN_LOGR_TILE, MAX_LOGR_TILE = 4, 9  # csrc/plan.hpp
+
+
+def check_pass_split(field: str, L: int, split: list[int]) -> None:
+    """The rules stntt_set_pass_split enforces (a rejected entry would make
+    every context creation fail, so they are checked at build time)."""
+    logc = 5 if field == "u32" else 4  # 128-byte rows of 4- or 8-byte words
+    assert 2 <= len(split) <= 4 and sum(split) == L, (field, L, split)
+    assert all(MIN_LOGR_TILE <= p <= MAX_LOGR_TILE for p in split), (field, L, split)
+    assert all(sum(split[:i]) >= logc for i in range(1, len(split))), (field, L, split)  # J >= C
+    assert all(L - p >= logc for p in split), (field, L, split)  # n / R >= C columns
+
+
+def check_batch_split(field: str, L: int, logs: list[int]) -> None:
+    """The rules of batch_validate_split (LDS-resident path)."""
+    assert field in ("u32", "goldilocks"), field
+    assert 1 <= len(logs) <= 8 and sum(logs) == L and all(1 <= x <= 4 for x in logs), (field, L, logs)
 
 
 def render(table: dict) -> str:
@@ -49,14 +66,14 @@ def render(table: dict) -> str:
     ]
     for field, entries in sorted(table.get("pass_split", {}).items(), key=lambda kv: FIELD_IDS[kv[0]]):
         for L, split in sorted(entries.items(), key=lambda kv: int(kv[0])):
-            assert 2 <= len(split) <= 4 and sum(split) == int(L), (field, L, split)
+            check_pass_split(field, int(L), split)
             padded = list(split) + [0] * (4 - len(split))
             lines.append(f"    {{{FIELD_IDS[field]}, {int(L)}, {len(split)}, {{{', '.join(map(str, padded))}}}}},"
                          f"  // {field}")
     lines += ["};", "", "inline constexpr TunedBatchSplit TUNED_BATCH_SPLITS[] = {"]
     for field, entries in sorted(table.get("batch_split", {}).items(), key=lambda kv: FIELD_IDS[kv[0]]):
         for L, logs in sorted(entries.items(), key=lambda kv: int(kv[0])):
-            assert 1 <= len(logs) <= 8 and sum(logs) == int(L) and all(1 <= x <= 15 for x in logs), (field, L, logs)
+            check_batch_split(field, int(L), logs)
             packed = 0
             for i, lr in enumerate(logs):
                 packed |= lr << (4 * i)

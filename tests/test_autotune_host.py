@@ -55,3 +55,20 @@ def test_compiled_table_matches_json():
         for L, split in entries.items():
             row = ", ".join(map(str, list(split) + [0] * (4 - len(split))))
             assert f"{{{G.FIELD_IDS[field]}, {L}, {len(split)}, {{{row}}}}}" in text, (field, L)
+
+
+def test_table_checks_reject_what_the_library_rejects():
+    import pytest
+    for field, L, split in (("goldilocks", 24, [8, 8, 8]), ("u32", 24, [9, 7, 8]), ("goldilocks", 13, [9, 4])):
+        G.check_pass_split(field, L, split)
+    for field, L, split in (("goldilocks", 24, [10, 7, 7]),  # radix above 2^9
+                            ("u32", 14, [4, 10]),            # radix above 2^9
+                            ("u32", 12, [4, 8]),             # second pass with J = 2^4 < 32 columns
+                            ("u32", 13, [9, 4]),             # n / R = 2^4 < 32 columns
+                            ("u32", 24, [8, 8, 7])):         # covers 2^23
+        with pytest.raises(AssertionError):
+            G.check_pass_split(field, L, split)
+    with pytest.raises(AssertionError):
+        G.check_batch_split("u32", 12, [5, 4, 3])  # sub-step radix above 2^4
+    with pytest.raises(AssertionError):
+        G.check_batch_split("u64", 12, [4, 4, 4])  # no batched 64-bit prime field
