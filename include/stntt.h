@@ -207,13 +207,17 @@ int stntt_batch_run_plain(stntt_batch *b, stntt_stream stream);
 int stntt_batch_destroy(stntt_batch *b);
 /* Tuning: radix split of the batched kernel's sub-steps for (field, log2n),
    4 bits per sub-step (first in the low bits), as chosen by the autotuner.
-   nsub = 0 restores the built-in split. */
+   Every context starts with the splits tuned on MI355X (compiled into the
+   library from stntt_amd/tuning/gfx950.json); this replaces one of them.
+   nsub = 0 removes the entry: the default split formula then applies. */
 int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t nsub, uint32_t radix_logs);
 /* Tuning: stages per tile pass of the multi-pass path for (field, log2n),
-   one log2 radix per pass (first pass first), as chosen by the autotuner;
-   npasses = 0 restores the built-in split.  Each pass covers 4..9 stages.
-   Applies to single transforms (batches == 1, what the autotuner times);
-   batches keep the built-in fewest-pass split. */
+   one log2 radix per pass (first pass first), as chosen by the autotuner.
+   Every context starts with the splits tuned on MI355X (compiled into the
+   library from stntt_amd/tuning/gfx950.json); this replaces one of them,
+   and npasses = 0 removes it (the fewest-pass default then applies).  Each
+   pass covers 4..9 stages.  Applies to single transforms (batches == 1,
+   what the autotuner times); batches keep the fewest-pass default. */
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr);
 
 /* ---- serving: continuous batching under an HBM budget -------------------
