@@ -72,3 +72,21 @@ def test_table_checks_reject_what_the_library_rejects():
         G.check_batch_split("u32", 12, [5, 4, 3])  # sub-step radix above 2^4
     with pytest.raises(AssertionError):
         G.check_batch_split("u64", 12, [4, 4, 4])  # no batched 64-bit prime field
+
+
+def test_measure_alternates_order_and_takes_medians():
+    """Every candidate runs once untimed, then rounds alternate the order, so
+    no candidate is always timed first (right after a pause, while the GPU
+    clock ramps up); the result is the per-candidate median."""
+    order = []
+    cur = {}
+    cands = [[8, 8, 8], [8, 9, 7], [9, 6, 9]]
+    med = A.measure(cands, lambda c: cur.update(c=str(c)), lambda: order.append(cur["c"]), lambda: None,
+                    reps=2, rounds=3, warm_s=0.0)
+    assert set(med) == {str(c) for c in cands} and all(v >= 0 for v in med.values())
+    names = [str(c) for c in cands]
+    assert len(order) == 3 + 3 * 3 * 2  # one untimed run each, then rounds x candidates x reps
+    assert order[:3] == names
+    timed = order[3:]
+    assert [timed[i] for i in range(0, len(timed), 2)] == names + names[::-1] + names
+    assert all(timed[i] == timed[i + 1] for i in range(0, len(timed), 2))
