@@ -128,7 +128,7 @@ def measure(cands, install, fn, sync, reps: int, rounds: int = 3, warm_s: float 
     return {k: float(np.median(v)) for k, v in acc.items()}
 
 
-def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True, fields=FIELDS,
+def tune(logs_pass=range(8, 27), logs_batch=range(10, 19), verbose: bool = True, fields=PASS_FIELDS,
          parts=("pass", "batch")) -> dict:
     import stntt_amd as S
     from .ntt import set_batch_split, set_pass_split
@@ -143,6 +143,9 @@ def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True
     for field in fields:
         logc = word_log2c(field)
         for L in logs_pass:
+            cands = pass_candidates(L, logc)
+            if not cands:  # below two tile passes: the single-pass kernel only
+                continue
             n = 1 << L
             if field == "u32":
                 root, mod = S.find_params(n, 11)
@@ -159,7 +162,6 @@ def tune(logs_pass=range(14, 27), logs_batch=range(10, 19), verbose: bool = True
                 fn = lambda: ctx.ntt_u64(buf, L, 1, root, mod)  # noqa: E731
             else:
                 fn = lambda: ctx.ntt_goldilocks(buf, L, 1)  # noqa: E731
-            cands = pass_candidates(L, logc)
             med = measure(cands, lambda c: set_pass_split(ctx, field, L, c), fn, ctx.sync, 20)
             times = {k: round(v * 1e6, 2) for k, v in med.items()}
             best = min(cands, key=lambda c: med[str(c)])

@@ -23,6 +23,10 @@ struct TunedBatchSplit {
 };
 
 inline constexpr TunedPassSplit TUNED_PASS_SPLITS[] = {
+    {0, 10, 2, {5, 5, 0, 0}},  // u32
+    {0, 11, 2, {6, 5, 0, 0}},  // u32
+    {0, 12, 2, {6, 6, 0, 0}},  // u32
+    {0, 13, 2, {6, 7, 0, 0}},  // u32
     {0, 14, 2, {7, 7, 0, 0}},  // u32
     {0, 15, 3, {6, 4, 5, 0}},  // u32
     {0, 16, 3, {6, 6, 4, 0}},  // u32
@@ -36,6 +40,12 @@ inline constexpr TunedPassSplit TUNED_PASS_SPLITS[] = {
     {0, 24, 3, {9, 7, 8, 0}},  // u32
     {0, 25, 3, {9, 9, 7, 0}},  // u32
     {0, 26, 3, {9, 9, 8, 0}},  // u32
+    {1, 8, 2, {4, 4, 0, 0}},  // goldilocks
+    {1, 9, 2, {5, 4, 0, 0}},  // goldilocks
+    {1, 10, 2, {6, 4, 0, 0}},  // goldilocks
+    {1, 11, 2, {5, 6, 0, 0}},  // goldilocks
+    {1, 12, 2, {6, 6, 0, 0}},  // goldilocks
+    {1, 13, 3, {5, 4, 4, 0}},  // goldilocks
     {1, 14, 3, {6, 4, 4, 0}},  // goldilocks
     {1, 15, 3, {6, 4, 5, 0}},  // goldilocks
     {1, 16, 3, {6, 4, 6, 0}},  // goldilocks
@@ -49,6 +59,12 @@ inline constexpr TunedPassSplit TUNED_PASS_SPLITS[] = {
     {1, 24, 3, {8, 8, 8, 0}},  // goldilocks
     {1, 25, 3, {8, 8, 9, 0}},  // goldilocks
     {1, 26, 3, {8, 9, 9, 0}},  // goldilocks
+    {2, 8, 2, {4, 4, 0, 0}},  // u64
+    {2, 9, 2, {4, 5, 0, 0}},  // u64
+    {2, 10, 2, {4, 6, 0, 0}},  // u64
+    {2, 11, 2, {6, 5, 0, 0}},  // u64
+    {2, 12, 2, {6, 6, 0, 0}},  // u64
+    {2, 13, 3, {4, 4, 5, 0}},  // u64
     {2, 14, 3, {4, 6, 4, 0}},  // u64
     {2, 15, 3, {4, 6, 5, 0}},  // u64
     {2, 16, 3, {6, 4, 6, 0}},  // u64

@@ -15,6 +15,8 @@ def test_pass_candidates_cover_every_stage():
             # every pass after the first starts at J >= the tile's columns
             assert all(sum(c[:i]) >= logc for i in range(1, len(c)))
         assert len(cands[0]) <= len(cands[-1])  # fewest passes first
+    # below two tile passes there is nothing to tune (the tuner skips these)
+    assert A.pass_candidates(9, 5) == [] and A.pass_candidates(7, 4) == []
 
 
 def test_merge_keeps_entries_not_retuned():
