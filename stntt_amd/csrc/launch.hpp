@@ -1,0 +1,42 @@
+// launch.hpp -- host-side launchers of the transform kernels.
+//
+// The pass kernels of each (field, direction) and the persistent batch
+// kernel are instantiated in translation units of their own
+// (launch_pass.hip, compiled once per field and direction, and
+// batch_kernel.hip), so the library's device code compiles in parallel.
+// Each launcher enqueues on stream `s` and returns without synchronising;
+// errors surface through hipGetLastError() as for a direct launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ntt_batch.hpp"
+#include "ntt_pass.hpp"
+
+namespace stntt {
+
+// One pass of a transform schedule (see plan.hpp):
+//   single: the whole transform in one LDS pass (R = n), `narrow` selecting
+//           the one-transform, one-wave variant;
+//   otherwise a tile pass of radix 2^logr; `first` (J = 1) / last pass are
+//           told apart by `first` and PassArgs::flags; `perm` stages a
+//           bit-reversed input (first pass) or output (last pass).
+template <class FP, int DIR>
+void launch_pass_dir(bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
+                     uint32_t grid, hipStream_t s);
+
+template <class FP>
+inline void launch_pass(int dir, bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
+                        uint32_t grid, hipStream_t s) {
+    if (dir == 0) launch_pass_dir<FP, 0>(single, first, perm, narrow, logr, a, grid, s);
+    else launch_pass_dir<FP, 1>(single, first, perm, narrow, logr, a, grid, s);
+}
+
+// The persistent descriptor-queue kernel (ntt_batch.hpp) and its counter
+// reset; occupancy of the former in workgroups per CU.
+void launch_batch_reset(uint32_t *counter, hipStream_t s);
+void launch_batch(uint32_t grid, const BatchTask *tasks, uint32_t count, uint32_t *counter, uint64_t *scratch,
+                  uint64_t scratch_stride, hipStream_t s);
+hipError_t batch_occupancy(int *per_cu);
+
+}  // namespace stntt

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/stntt.h"
+#include "launch.hpp"
 #include "ntt_batch.hpp"
 #include "ntt_pass.hpp"
 #include "plan.hpp"
@@ -279,62 +280,6 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
     *out = plan.get();
     ctx->plans.emplace(key, std::move(plan));
     return STNTT_OK;
-}
-
-// ---------------------------------------------------------------------------
-// kernel dispatch: one template instance per (field, dir, LOGR, single)
-// ---------------------------------------------------------------------------
-template <class FP, int DIR, int LOGR>
-static void launch_single(const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s) {
-    constexpr int LOGC = pass_logc<FP>(LOGR, true);
-    if constexpr (LOGR <= NARROW_MAX_LOGR) {
-        if (narrow) {  // one transform, one wave
-            hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, 0, NARROW_THREADS, DIR, true>), dim3(grid),
-                               dim3(NARROW_THREADS), 0, s, a);
-            return;
-        }
-    }
-    hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, LOGC, PASS_THREADS, DIR, true>), dim3(grid), dim3(PASS_THREADS), 0,
-                       s, a);
-}
-
-template <class FP, int DIR, int LOGR>
-static void launch_tile(bool first, bool perm, const PassArgs<FP> &a, uint32_t grid, hipStream_t s) {
-    constexpr int NT = tile_threads<FP, LOGR>();
-    // perm: bit-reversed input on the first pass, bit-reversed output on the last
-    if (first) {
-        if (perm) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true, PERM_IN>), dim3(grid), dim3(NT), 0, s, a);
-        else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, true>), dim3(grid), dim3(NT), 0, s, a);
-    } else {
-        if (perm) hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false, PERM_OUT>), dim3(grid), dim3(NT), 0, s, a);
-        else hipLaunchKernelGGL((ntt_tile_kernel<FP, LOGR, DIR, false>), dim3(grid), dim3(NT), 0, s, a);
-    }
-}
-
-template <class FP, int DIR, int... L>
-static void launch_single_impl(int logr, const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s,
-                               std::integer_sequence<int, L...>) {
-    ((logr == L + 1 ? (launch_single<FP, DIR, L + 1>(a, grid, narrow, s), 0) : 0), ...);
-}
-
-template <class FP, int DIR, int... L>
-static void launch_tile_impl(int logr, bool first, bool perm, const PassArgs<FP> &a, uint32_t grid, hipStream_t s,
-                             std::integer_sequence<int, L...>) {
-    ((logr == L + MIN_LOGR_TILE ? (launch_tile<FP, DIR, L + MIN_LOGR_TILE>(first, perm, a, grid, s), 0) : 0), ...);
-}
-
-template <class FP>
-static void launch_pass(int dir, bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
-                        uint32_t grid, hipStream_t s) {
-    using SeqS = std::make_integer_sequence<int, MAX_LOGR_SINGLE<FP>>;
-    using SeqT = std::make_integer_sequence<int, MAX_LOGR_TILE - MIN_LOGR_TILE + 1>;
-    if (dir == 0) {
-        if (single) launch_single_impl<FP, 0>(logr, a, grid, narrow, s, SeqS{});
-        else launch_tile_impl<FP, 0>(logr, first, perm, a, grid, s, SeqT{});
-    } else {
-        if (single) launch_single_impl<FP, 1>(logr, a, grid, narrow, s, SeqS{});
-        else launch_tile_impl<FP, 1>(logr, first, perm, a, grid, s, SeqT{});
-    }
 }
 
 template <class FP>
@@ -660,7 +605,7 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
         return cx > cy;
     });
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ntt_batch_kernel, BATCH_THREADS, 0));
+    HIP_TRY(batch_occupancy(&per_cu));
     if (per_cu < 1) per_cu = 1;
     b->count = (uint32_t)tasks.size();
     b->grid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * per_cu, std::max<size_t>(tasks.size(), 1));
@@ -686,11 +631,10 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
 }
 
 static int batch_launch(stntt_batch *b, hipStream_t s) {
-    hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, b->d_counter);
+    launch_batch_reset(b->d_counter, s);
     HIP_TRY(hipGetLastError());
     if (b->count) {
-        hipLaunchKernelGGL(ntt_batch_kernel, dim3(b->grid), dim3(BATCH_THREADS), 0, s, b->d_tasks, b->count,
-                           b->d_counter, b->d_scratch, b->scratch_stride);
+        launch_batch(b->grid, b->d_tasks, b->count, b->d_counter, b->d_scratch, b->scratch_stride, s);
         HIP_TRY(hipGetLastError());
     }
     for (const BatchGroup &g : b->groups) {
@@ -837,9 +781,9 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
     };
     if (nb) {
         HIP_TRY(hipMemcpyAsync(W.d_tasks, W.h_tasks, nb * sizeof(BatchTask), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(ntt_batch_reset, dim3(1), dim3(64), 0, s, W.d_counter);
-        hipLaunchKernelGGL(ntt_batch_kernel, dim3(std::min<uint32_t>(nb, SERVER_BATCH_GRID)), dim3(BATCH_THREADS), 0, s,
-                           W.d_tasks, nb, W.d_counter, W.d_scratch, (uint64_t)1 << SERVER_BATCH_MAX_LOG);
+        launch_batch_reset(W.d_counter, s);
+        launch_batch(std::min<uint32_t>(nb, SERVER_BATCH_GRID), W.d_tasks, nb, W.d_counter, W.d_scratch,
+                     (uint64_t)1 << SERVER_BATCH_MAX_LOG, s);
         HIP_TRY(hipGetLastError());
         std::vector<ServerRequest *> reqs;
         for (ServerRequest *r : round)
