@@ -168,6 +168,33 @@ int stntt_auto_ntt_i32_ex(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, ui
 int stntt_auto_polymul_i32(stntt_ctx *ctx, int32_t *host_a, int32_t *host_b, uint32_t n, uint32_t root,
                            uint32_t mod, float *kernel_ms);
 
+/* ---- the reference's stage loop, truncated or index-only ----------------
+   nttStockham with nsteps and nttStockhamIdx (src/ntt/ntt_cpu.cpp:46-127):
+   the reference's radix-2 Stockham loop run stage by stage on the device (a
+   launch per stage over all rows), for the first nsteps stages (nsteps < 0
+   or 0: none, the data untouched; at most 30).  Stage i uses the
+   reference's stride J = pow(2, i, mod) (:68, :105) -- 2^i below the
+   transform size -- and its zero-initialised ping-pong buffer, so calls
+   with nsteps past log2 n give the reference's words too (the reference
+   divides by J, so a zero stride is STNTT_EINVAL).  Arithmetic is exact in
+   Z_mod for any root and any 2 <= mod < 2^31 (the reference's int products
+   wrap from mod = 46341 on); with nsteps = log2 n the words equal the full
+   transform's.  This is a diagnostic path: one HBM pass per stage, where
+   stntt_ntt_u32 runs 2-3.  The Idx form moves 4-byte words only (mod
+   selects the strides): with nsteps = log2 n it is the bit-reversal
+   permutation.  flags: STNTT_SIGNED_INPUT (int32 data, reduced with the
+   reference's modulo()). */
+int stntt_stockham_steps_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t root,
+                             uint32_t mod, int nsteps, uint32_t flags, stntt_stream stream);
+int stntt_stockham_idx_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t mod,
+                           int nsteps, stntt_stream stream);
+/* the same on host int vectors of n * batches words, autoNtt-shaped (copy
+   in, run, copy back; kernel-only milliseconds in *kernel_ms) */
+int stntt_auto_stockham_steps_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                                  uint32_t root, uint32_t mod, int nsteps, float *kernel_ms);
+int stntt_auto_stockham_idx_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                                uint32_t mod, int nsteps, float *kernel_ms);
+
 /* ---- batched mixed transforms -------------------------------------------
    Thousands of independent transforms of mixed size (log2n <= 24), field and
    direction.  Transforms that fit a workgroup's LDS (32 KiB: 2^12
@@ -241,14 +268,18 @@ typedef struct stntt_server_stats {
     uint64_t completed;
     double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result in host_out */
     double span_s;                          /* first submit -> last completion */
-    uint64_t peak_bytes;                    /* peak device bytes admitted */
+    uint64_t peak_bytes;                    /* peak request bytes held: admitted + cached, <= budget */
     uint64_t rounds;                        /* admission rounds launched */
     uint64_t batched;                       /* requests run through the batch kernel */
+    uint64_t held_bytes;                    /* request bytes held now (admitted + cached) */
 } stntt_server_stats;
 
 typedef struct stntt_server stntt_server;
-/* workers: concurrent streams (1..16).  The budget covers admitted request
-   buffers plus a fixed per-worker reservation for scratch. */
+/* workers: concurrent streams (1..16).  The budget covers a fixed
+   per-worker reservation for scratch plus every request buffer the server
+   holds: admitted ones and idle cached ones (released, other size classes
+   first, when a new allocation needs the room).  A request the device
+   cannot hold even with nothing else in flight fails with STNTT_ENOMEM. */
 int stntt_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, stntt_server **out);
 int stntt_server_destroy(stntt_server *srv);
 int stntt_server_host_alloc(stntt_server *srv, size_t bytes, void **host_ptr);

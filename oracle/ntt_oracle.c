@@ -157,17 +157,22 @@ int orc_twiddles_u32(uint32_t *out, uint64_t n, uint32_t root, uint32_t mod) {
 }
 
 /* Stockham DIF, natural order in and out: src/ntt/ntt_cpu.cpp:46-85.
-   Stage i: J = 2^i; for s < n/(2J), w = T[J*s]; for j < J:
+   Stage i: J = pow(2, i, mod) (:68; 2^i while 2^i < mod, so for every
+   stage below log2 n); for s < n/(2J), w = T[J*s]; for j < J:
      y[2sJ+j]     = x[sJ+j] + x[sJ+j+n/2]
      y[(2s+1)J+j] = w * (x[sJ+j] - x[sJ+j+n/2])
-   then swap x and y; an odd number of swaps copies back (:81-84).
-   Inputs are reduced mod p first (the reference's modulo() accepts any int). */
+   then swap x and y; an odd number of swaps copies back (:81-84).  y starts
+   zeroed (:52) and positions a stage does not write keep their words, which
+   matters only for nsteps past log2 n.  Operands are reduced mod p as they
+   are read (the reference's modulo() accepts any int). */
 int orc_stockham_u32(uint32_t *vec, uint64_t n, uint32_t root, uint32_t mod, int nsteps) {
     if (!is_pow2(n)) return -1;
     if (mod < 2) return -2;
     int steps = (int)ilog2(n);
     if (nsteps >= 0) steps = nsteps;
-    uint32_t *y = (uint32_t *)malloc(n * sizeof(uint32_t));
+    for (int i = 0; i < steps; i++)
+        if (orc_powmod(2, (uint64_t)i, mod) == 0) return -2; /* the reference divides by J */
+    uint32_t *y = (uint32_t *)calloc(n, sizeof(uint32_t));
     uint32_t *T = (uint32_t *)malloc((n / 2 + 1) * sizeof(uint32_t));
     if (!y || !T) {
         free(y);
@@ -176,15 +181,12 @@ int orc_stockham_u32(uint32_t *vec, uint64_t n, uint32_t root, uint32_t mod, int
     }
     orc_twiddles_u32(T, n, root, mod);
     uint32_t *x = vec, *orig = vec;
-    /* with zero stages the reference returns its input untouched */
-    if (steps > 0)
-        for (uint64_t i = 0; i < n; ++i) x[i] %= mod;
     for (int i = 0; i < steps; i++) {
-        uint64_t J = 1ull << i;
+        uint64_t J = orc_powmod(2, (uint64_t)i, mod);
         for (uint64_t s = 0; s < n / (2 * J); s++) {
             uint32_t w = T[J * s];
             for (uint64_t j = 0; j < J; ++j) {
-                uint32_t l = x[s * J + j], r = x[s * J + j + n / 2];
+                uint32_t l = x[s * J + j] % mod, r = x[s * J + j + n / 2] % mod;
                 uint32_t sum = (uint32_t)(((uint64_t)l + r) % mod);
                 uint32_t dif = (uint32_t)(((uint64_t)l + mod - r) % mod);
                 y[2 * s * J + j] = sum;
@@ -377,16 +379,20 @@ int orc_stockham_refcompat(int32_t *vec, uint64_t n, int32_t root, int32_t mod, 
     return 0;
 }
 
-/* data movement of the Stockham without arithmetic: src/ntt/ntt_cpu.cpp:92-122 */
-int orc_stockham_idx(int32_t *vec, uint64_t n, int nsteps) {
+/* data movement of the Stockham without arithmetic: src/ntt/ntt_cpu.cpp:92-122,
+   stage strides J = pow(2, i, mod) (:105) and a zeroed y (:98) */
+int orc_stockham_idx_mod(int32_t *vec, uint64_t n, uint32_t mod, int nsteps) {
     if (!is_pow2(n)) return -1;
+    if (mod < 2) return -2;
     int steps = (int)ilog2(n);
     if (nsteps >= 0) steps = nsteps;
+    for (int i = 0; i < steps; i++)
+        if (orc_powmod(2, (uint64_t)i, mod) == 0) return -2; /* the reference divides by J */
     int32_t *y = (int32_t *)calloc(n, sizeof(int32_t));
     if (!y) return -2;
     int32_t *x = vec, *orig = vec;
     for (int i = 0; i < steps; i++) {
-        uint64_t J = 1ull << i;
+        uint64_t J = orc_powmod(2, (uint64_t)i, mod);
         for (uint64_t s = 0; s < n / (2 * J); s++)
             for (uint64_t j = 0; j < J; ++j) {
                 y[2 * s * J + j] = x[s * J + j];
@@ -400,6 +406,9 @@ int orc_stockham_idx(int32_t *vec, uint64_t n, int nsteps) {
     free(x == orig ? y : x);
     return 0;
 }
+
+/* with the modulus the reference's own fixtures used (2^31 - 1 > n: J = 2^i) */
+int orc_stockham_idx(int32_t *vec, uint64_t n, int nsteps) { return orc_stockham_idx_mod(vec, n, 2147483647u, nsteps); }
 
 /* ---------------------------------------------------------------------- */
 /* Goldilocks, exact.  The reference declares the field                   */

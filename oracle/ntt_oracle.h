@@ -60,7 +60,9 @@ int orc_pwc_ntt_u32(uint32_t *a, uint32_t *b, uint64_t n, uint32_t root, uint32_
 /* ---- reference-compatible int32 wrap mode ------------------------------ */
 /* Stockham exactly as ntt_cpu.cpp:46-85 with int32 overflow made explicit */
 int orc_stockham_refcompat(int32_t *vec, uint64_t n, int32_t root, int32_t mod, int nsteps);
-/* index-only Stockham (ntt_cpu.cpp:92-122) */
+/* index-only Stockham (ntt_cpu.cpp:92-122), strides pow(2, i, mod) */
+int orc_stockham_idx_mod(int32_t *vec, uint64_t n, uint32_t mod, int nsteps);
+/* the same with mod = 2^31 - 1 (strides 2^i) */
 int orc_stockham_idx(int32_t *vec, uint64_t n, int nsteps);
 
 /* ---- Goldilocks p = 2^64 - 2^32 + 1, exact ----------------------------- */

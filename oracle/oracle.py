@@ -62,6 +62,7 @@ def lib() -> ctypes.CDLL:
         L.orc_pwc_ntt_u32.argtypes = [_u32p, _u32p, _U64, _U32, _U32]
         L.orc_stockham_refcompat.argtypes = [_i32p, _U64, _c.c_int32, _c.c_int32, _c.c_int]
         L.orc_stockham_idx.argtypes = [_i32p, _U64, _c.c_int]
+        L.orc_stockham_idx_mod.argtypes = [_i32p, _U64, _U32, _c.c_int]
         for f in ("orc_gl_mul", "orc_gl_add", "orc_gl_sub", "orc_gl_pow"):
             getattr(L, f).argtypes = [_U64, _U64]
             getattr(L, f).restype = _U64
@@ -101,6 +102,8 @@ def ref() -> ctypes.CDLL:
         R.ref_find_params.argtypes = [_U64, _U64, _c.POINTER(_U64), _c.POINTER(_U64)]
         R.ref_ntt_stockham.argtypes = [_i32p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int]
         R.ref_ntt_stockham_idx.argtypes = [_i32p, _c.c_int, _c.c_int, _c.c_int]
+        R.ref_ntt_stockham_idx_mod.argtypes = [_i32p, _c.c_int, _c.c_int, _c.c_int, _c.c_int]
+        R.ref_elem_wise_mul.argtypes = [_i32p, _i32p, _c.c_int, _U64]
         R.ref_ntt_ct.argtypes = [_i32p, _c.c_int, _c.c_int, _c.c_int]
         R.ref_intt_gs.argtypes = [_i32p, _c.c_int, _c.c_int, _c.c_int]
         R.ref_pwc_convolution.argtypes = [_i32p, _i32p, _c.c_int, _i32p, _c.POINTER(_c.c_int),
@@ -177,9 +180,9 @@ def stockham_refcompat(x: np.ndarray, n: int, root: int, mod: int, batches: int 
     return v
 
 
-def stockham_idx(x: np.ndarray, n: int, nsteps: int = -1) -> np.ndarray:
+def stockham_idx(x: np.ndarray, n: int, nsteps: int = -1, mod: int = 2**31 - 1) -> np.ndarray:
     v = np.ascontiguousarray(x, dtype=np.int32).copy()
-    _check(lib().orc_stockham_idx(v, n, nsteps), "stockham_idx")
+    _check(lib().orc_stockham_idx_mod(v, n, mod, nsteps), "stockham_idx")
     return v
 
 

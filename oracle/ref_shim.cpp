@@ -58,6 +58,23 @@ int ref_ntt_stockham_idx(int32_t *vec, int size, int batches, int nsteps) {
     }
 }
 
+// nttStockhamIdx with the caller's modulus, which sets the stage strides
+// J = pow(2, i, mod) (src/ntt/ntt_cpu.cpp:105)
+int ref_ntt_stockham_idx_mod(int32_t *vec, int size, int batches, int mod, int nsteps) {
+    try {
+        nttStockhamIdx(std::span<int>(vec, (size_t)size * batches), size, 0, mod, batches, steps_opt(nsteps));
+        return 0;
+    } catch (const std::invalid_argument &) {
+        return -1;
+    }
+}
+
+// elemWiseMul<int>(s0, s1, mod): src/ntt/ntt_util.h:28-33
+int ref_elem_wise_mul(int32_t *s0, int32_t *s1, int size, uint64_t mod) {
+    elemWiseMul(std::span<int>(s0, (size_t)size), std::span<int>(s1, (size_t)size), (size_t)mod);
+    return 0;
+}
+
 // nttCt(vector&, root, mod): src/ntt/ntt_cpu.cpp:11-42
 int ref_ntt_ct(int32_t *vec, int size, int root, int mod) {
     std::vector<int> v(vec, vec + size);

@@ -61,6 +61,11 @@ SIGNATURES = [
     ("stntt_auto_ntt_goldilocks", _int, [_vp, _vp, _sz, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
     ("stntt_auto_ntt_i32_ex", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _u32, _int, _u32, _c.POINTER(_c.c_float)]),
     ("stntt_auto_polymul_i32", _int, [_vp, _vp, _vp, _u32, _u32, _u32, _c.POINTER(_c.c_float)]),
+    ("stntt_stockham_steps_u32", _int, [_vp, _vp, _u32, _u64, _u32, _u32, _int, _u32, _vp]),
+    ("stntt_stockham_idx_u32", _int, [_vp, _vp, _u32, _u64, _u32, _int, _vp]),
+    ("stntt_auto_stockham_steps_i32", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _u32, _int,
+                                             _c.POINTER(_c.c_float)]),
+    ("stntt_auto_stockham_idx_i32", _int, [_vp, _vp, _sz, _u32, _u32, _u32, _int, _c.POINTER(_c.c_float)]),
     ("stntt_batch_create", _int, [_vp, _vp, _sz, _c.POINTER(_vp)]),
     ("stntt_batch_run", _int, [_vp, _vp]),
     ("stntt_batch_capture", _int, [_vp, _vp]),
@@ -109,7 +114,7 @@ class ServerStats(_c.Structure):
     """stntt_server_stats (include/stntt.h)"""
     _fields_ = [("completed", _u64), ("p50_ms", _c.c_double), ("p99_ms", _c.c_double), ("mean_ms", _c.c_double),
                 ("max_ms", _c.c_double), ("span_s", _c.c_double), ("peak_bytes", _u64), ("rounds", _u64),
-                ("batched", _u64)]
+                ("batched", _u64), ("held_bytes", _u64)]
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -57,6 +57,16 @@ def apply_table(ctx, path: str = TABLE) -> int:
     return k
 
 
+def table_pass_split(field: str, L: int, path: str = TABLE) -> list[int] | None:
+    """The tuned stages-per-pass split of (field, 2^L) in the table, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    v = table.get("pass_split", {}).get(field, {}).get(str(L))
+    return list(v) if v else None
+
+
 def pass_candidates(L: int, logc: int, lo: int = 4, hi: int = 9) -> list[list[int]]:
     out = []
     for np_ in (2, 3, 4):

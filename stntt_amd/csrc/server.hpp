@@ -19,8 +19,13 @@
 //      as its own kernel is done.  Uploads, kernels and downloads of
 //      different rounds overlap;
 //   3. retires each group of a round (the batched small requests, or one
-//      large request) when its download event has fired, returns its bytes
-//      to the budget and its buffers to the cache, and wakes waiters.
+//      large request) when its download event has fired, moves its buffers
+//      to the cache and wakes waiters.
+// The budget covers every device byte held for requests: admitted buffers
+// plus cached idle ones.  A request of a size class with no cached buffer
+// first releases cached buffers of other classes until its allocation fits;
+// one the device cannot hold with nothing else in flight fails with
+// STNTT_ENOMEM rather than waiting for memory that will never free up.
 // Requests arriving while others run are admitted as soon as budget frees
 // up: continuous batching, not a queue drained one request at a time.
 #pragma once
@@ -101,7 +106,8 @@ struct stntt_server {
     std::vector<double> latencies;
     uint64_t next_id = 1;
     uint64_t completed = 0;
-    uint64_t peak_used = 0;
+    uint64_t cached = 0;     // bytes of idle buffers in `cache` (held, charged to the budget)
+    uint64_t peak_used = 0;  // peak of used + cached
     uint64_t rounds_launched = 0, batched_requests = 0;
     bool stop = false;
     std::thread thread;

@@ -288,6 +288,145 @@ __global__ void copy_kernel(const typename FP::T *in, typename FP::T *out, uint6
     if (i < count) out[i] = in[i];
 }
 
+// The context's scratch buffer, grown to at least `bytes`, for work on
+// stream s.  Work on different streams shares it: the previous user's
+// completion (scratch_ev) is waited for on s first.  release_scratch marks
+// the end of this use.  Graph captures skip the cross-stream ordering.
+static int acquire_scratch(stntt_ctx *ctx, size_t bytes, hipStream_t s, void **out) {
+    int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, bytes);
+    if (rc) return rc;
+    if (ctx->scratch_last && ctx->scratch_last != s && !stream_capturing(s))
+        HIP_TRY(hipStreamWaitEvent(s, ctx->scratch_ev, 0));
+    *out = ctx->scratch;
+    return STNTT_OK;
+}
+
+static int release_scratch(stntt_ctx *ctx, hipStream_t s) {
+    if (stream_capturing(s)) return STNTT_OK;
+    HIP_TRY(hipEventRecord(ctx->scratch_ev, s));
+    ctx->scratch_last = s;
+    return STNTT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the reference's radix-2 stage loop, stage by stage (nsteps, nttStockhamIdx)
+// ---------------------------------------------------------------------------
+// One stage of src/ntt/ntt_cpu.cpp:66-79 (ARITH) or of its index-only twin
+// :103-118 (!ARITH) with stride J, from `in` to `out`, one thread per pair
+// (row b, s, j) with s < n / (2J), j < J:
+//     out[2sJ + j] = l + r,  out[(2s + 1)J + j] = w^(Js) (l - r)
+//     l = in[sJ + j], r = in[sJ + j + n/2]
+// (or the plain moves).  J is the reference's pow(2, i, mod), which equals
+// 2^i below the transform size but may be anything past it, so the strides
+// are general; positions of `out` the stage does not write keep their
+// contents, as in the reference's ping-pong.  Arithmetic is exact in the
+// 32-bit field: inputs are reduced with the reference's modulo() semantics
+// as they are read, twiddles w^e by square-and-multiply (a diagnostic path,
+// so no table is needed and any root and any modulus 2 <= p < 2^31 work).
+template <bool ARITH>
+__global__ void stockham_stage_kernel(const uint32_t *in, uint32_t *out, uint32_t log_n, uint32_t J, uint32_t pairs,
+                                      uint64_t total, uint32_t root, uint32_t p, uint32_t bar, uint32_t c32,
+                                      uint64_t mu, int is_signed) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t b = i / pairs;
+    const uint32_t q = (uint32_t)(i - b * pairs);
+    const uint32_t s = q / J, j = q - s * J;
+    const uint64_t base = b << log_n;
+    const uint32_t half = (1u << log_n) >> 1;
+    uint32_t l = in[base + (uint64_t)s * J + j];
+    uint32_t r = in[base + (uint64_t)s * J + j + half];
+    if constexpr (ARITH) {
+        l = u32_reduce_input(l, p, bar, c32, is_signed != 0);
+        r = u32_reduce_input(r, p, bar, c32, is_signed != 0);
+        uint32_t w = 1 % p, bw = root;  // root < p
+        for (uint32_t e = J * s; e; e >>= 1) {  // J s < n / 2
+            if (e & 1) w = u32_mul_any(w, bw, p, mu);
+            bw = u32_mul_any(bw, bw, p, mu);
+        }
+        out[base + 2ull * s * J + j] = u32_add(l, r, p);
+        out[base + (2ull * s + 1) * J + j] = u32_mul_any(u32_sub(l, r, p), w, p, mu);
+    } else {
+        out[base + 2ull * s * J + j] = l;
+        out[base + (2ull * s + 1) * J + j] = r;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *out, uint64_t count, uint32_t value) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = value;
+}
+
+// nsteps stages of the reference's loop over d_data[batches][2^log_n], in
+// place from the caller's view: a zeroed ping-pong buffer, a launch per
+// stage, then a copy back when the result ended in the other buffer (the
+// reference's odd-swap copy, src/ntt/ntt_cpu.cpp:81-84).
+// stages a nsteps call may ask for: pow(2, i, mod) is exact in the
+// reference's int arithmetic up to i = 30
+constexpr int MAX_STAGE_STEPS = 30;
+
+template <bool ARITH>
+static int stockham_steps(stntt_ctx *ctx, uint32_t *d_data, uint32_t log_n, uint64_t batches, uint32_t root,
+                          uint32_t mod, int nsteps, bool is_signed, hipStream_t s) {
+    const uint64_t n = 1ull << log_n, total_words = batches << log_n;
+    if (batches == 0 || nsteps <= 0) return STNTT_OK;  // no stage: the input as it is
+    if (total_words >= (1ull << 40)) return fail(STNTT_EINVAL, "buffer too large");
+    for (int i = 0; i < nsteps; ++i)  // the reference divides by J (src/ntt/ntt_cpu.cpp:69)
+        if (powmod(2, (uint64_t)i, mod) == 0)
+            return fail(STNTT_EINVAL, "stage %d has stride pow(2, %d, %u) = 0", i, i, mod);
+    void *scratch = nullptr;
+    int rc = acquire_scratch(ctx, total_words * 4, s, &scratch);
+    if (rc) return rc;
+    auto blocks = [](uint64_t count) { return dim3((uint32_t)((count + 255) / 256)); };
+    hipLaunchKernelGGL(fill_u32_kernel, blocks(total_words), dim3(256), 0, s, (uint32_t *)scratch, total_words, 0u);
+    HIP_TRY(hipGetLastError());
+    uint32_t *cur = d_data, *other = (uint32_t *)scratch;
+    const uint32_t p = mod;
+    const uint32_t bar = (uint32_t)((1ull << 32) / p), c32 = (uint32_t)((1ull << 32) % p);
+    const uint64_t mu = ~0ull / p;
+    for (int i = 0; i < nsteps; ++i) {
+        const uint64_t J = powmod(2, (uint64_t)i, mod);  // pow(2, i, mod), src/ntt/ntt_cpu.cpp:68
+        const uint64_t pairs = J * (n / (2 * J));
+        if (pairs) {
+            const uint64_t count = batches * pairs;
+            hipLaunchKernelGGL(stockham_stage_kernel<ARITH>, blocks(count), dim3(256), 0, s, cur, other, log_n,
+                               (uint32_t)J, (uint32_t)pairs, count, root % p, p, bar, c32, mu, is_signed ? 1 : 0);
+            HIP_TRY(hipGetLastError());
+        }
+        std::swap(cur, other);
+    }
+    if (cur != d_data) {
+        hipLaunchKernelGGL(copy_kernel<FieldU32>, blocks(total_words), dim3(256), 0, s, cur, d_data, total_words);
+        HIP_TRY(hipGetLastError());
+    }
+    return release_scratch(ctx, s);
+}
+
+// The pass schedule of `batches` transforms of 2^L words: the fewest-pass
+// default, or the context's tuned split for a single transform (tuned
+// splits were measured on one transform, i.e. latency; a batch is
+// throughput-bound and keeps the default).
+template <class FP>
+static Schedule transform_schedule(const stntt_ctx *ctx, int field, uint32_t L, uint64_t batches) {
+    Schedule sch = make_schedule<FP>((int)L, batches);
+    if (!sch.single && batches == 1) {
+        auto it = ctx->pass_split.find({field, L});
+        if (it != ctx->pass_split.end()) {
+            sch.npasses = (int)it->second.size();
+            for (int i = 0; i < sch.npasses; ++i) sch.logr[i] = it->second[i];
+        }
+    }
+    return sch;
+}
+
+// ping-pong bytes a schedule needs besides the data (no copy-back: the last
+// pass writes the caller's buffer)
+template <class FP>
+static uint64_t schedule_scratch_bytes(const Schedule &sch, uint64_t total_words) {
+    if (sch.npasses < 2) return 0;
+    return (sch.npasses > 2 ? 2 : 1) * total_words * sizeof(typename FP::T);
+}
+
 // Runs the pass schedule of `plan` over d_data[batches][n] in place.
 // flags: STNTT_SIGNED_INPUT, STNTT_BITREV_IN, STNTT_BITREV_OUT.  mul_in, when
 // set, is multiplied into the input word by word as the first pass loads it
@@ -310,16 +449,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         return fail(STNTT_EINVAL, "%llu transforms of 2^%u words is more than one call handles",
                     (unsigned long long)batches, L);
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
-    Schedule sch = make_schedule<FP>(L, batches);
-    // tuned splits were measured on one transform (latency); a batch is
-    // throughput-bound and keeps the fewest-pass default
-    if (!sch.single && batches == 1) {
-        auto it = ctx->pass_split.find({plan.key.field, L});
-        if (it != ctx->pass_split.end()) {
-            sch.npasses = (int)it->second.size();
-            for (int i = 0; i < sch.npasses; ++i) sch.logr[i] = it->second[i];
-        }
-    }
+    const Schedule sch = transform_schedule<FP>(ctx, plan.key.field, L, batches);
     if (!sch.single)
         for (int p = 0; p < sch.npasses; ++p) {
             const uint64_t tiles = batches * ((((1ull << L) >> sch.logr[p])) >> tile_logc<FP>());
@@ -335,17 +465,15 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
-        const int halves = sch.npasses > 2 ? 2 : 1;
         if (ext_scratch) {
             scratch[0] = ext_scratch;
         } else {
-            int rc = ensure_buffer(&ctx->scratch, &ctx->scratch_bytes, halves * total * sizeof(T));
+            void *p = nullptr;
+            int rc = acquire_scratch(ctx, schedule_scratch_bytes<FP>(sch, total), s, &p);
             if (rc) return rc;
-            scratch[0] = (T *)ctx->scratch;
-            if (ctx->scratch_last && ctx->scratch_last != s && !stream_capturing(s))
-                HIP_TRY(hipStreamWaitEvent(s, ctx->scratch_ev, 0));
+            scratch[0] = (T *)p;
         }
-        scratch[1] = halves > 1 ? scratch[0] + total : nullptr;
+        scratch[1] = sch.npasses > 2 ? scratch[0] + total : nullptr;
     }
     PassArgs<FP> a{};
     a.hi = (const typename FP::Tw *)plan.hi();
@@ -403,10 +531,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         src = dst;
         log_j += logr;
     }
-    if (sch.npasses > 1 && !ext_scratch && !stream_capturing(s)) {
-        HIP_TRY(hipEventRecord(ctx->scratch_ev, s));
-        ctx->scratch_last = s;
-    }
+    if (sch.npasses > 1 && !ext_scratch) return release_scratch(ctx, s);
     return STNTT_OK;
 }
 
@@ -690,16 +815,52 @@ static uint64_t server_worker_reserve(uint32_t max_log) {
 }
 constexpr uint32_t SERVER_MAX_LOG = 22;
 
+// Frees cached (idle) device buffers, other size classes than `keep`
+// first, until the bytes held by the server (admitted requests plus the
+// cache) are at most `target`.  A cached buffer belongs to no round in
+// flight, so freeing it races with nothing.
+static void server_trim_cache(stntt_server *S, uint64_t target, size_t keep) {
+    for (int pass = 0; pass < 2 && S->used + S->cached > target; ++pass) {
+        for (auto it = S->cache.begin(); it != S->cache.end() && S->used + S->cached > target; ++it) {
+            if (pass == 0 && it->first == keep) continue;
+            auto &v = it->second;
+            while (!v.empty() && S->used + S->cached > target) {
+                (void)hipFree(v.back());
+                v.pop_back();
+                S->cached -= it->first;
+            }
+        }
+    }
+}
+
+// A device buffer of `bytes` for a request being admitted (the caller has
+// checked used + bytes <= budget).  A cached buffer of the size class is
+// reused; otherwise cached buffers of other classes are released until the
+// new one fits the budget beside everything held, and on an allocation
+// failure the whole cache is released and the allocation tried once more.
+// Null when the device cannot take it now.
 static void *server_get_buffer(stntt_server *S, size_t bytes) {
     auto &v = S->cache[bytes];
     if (!v.empty()) {
         void *p = v.back();
         v.pop_back();
+        S->cached -= bytes;
         return p;
     }
+    server_trim_cache(S, S->budget - bytes, bytes);
     void *p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    return p;
+    if (hipMalloc(&p, bytes) == hipSuccess) return p;
+    server_trim_cache(S, 0, 0);
+    if (hipMalloc(&p, bytes) == hipSuccess) return p;
+    return nullptr;
+}
+
+// a retired request's buffer goes back to the cache (still held)
+static void server_cache_put(stntt_server *S, ServerRequest &r) {
+    S->used -= r.cost;
+    S->cache[r.d_bytes].push_back(r.d_buf);
+    S->cached += r.d_bytes;
+    r.d_buf = nullptr;
 }
 
 // Launch one admission round on worker w.  Runs on the server thread with
@@ -821,9 +982,7 @@ static void server_loop(stntt_server *S) {
                     r.done = true;
                     r.latency_ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
                     S->latencies.push_back(r.latency_ms);
-                    S->used -= r.cost;
-                    S->cache[r.d_bytes].push_back(r.d_buf);
-                    r.d_buf = nullptr;
+                    server_cache_put(S, r);
                     S->completed++;
                 }
                 S->t_last = now;
@@ -856,10 +1015,19 @@ static void server_loop(stntt_server *S) {
                 ServerRequest &r = S->reqs[S->pending.front()];
                 if (S->used + r.cost > S->budget) break;  // wait for memory to free up
                 void *buf = server_get_buffer(S, r.d_bytes);
-                if (!buf) break;
+                if (!buf) {
+                    if (!round.empty() || !S->rounds.empty()) break;  // memory frees up as rounds retire
+                    // nothing in flight and the whole cache released: the
+                    // device cannot hold this request now, fail it
+                    r.done = true;
+                    r.status = STNTT_ENOMEM;
+                    S->pending.pop_front();
+                    S->cv_done.notify_all();
+                    continue;
+                }
                 r.d_buf = buf;
                 S->used += r.cost;
-                S->peak_used = std::max(S->peak_used, S->used);
+                S->peak_used = std::max(S->peak_used, S->used + S->cached);
                 round.push_back(&r);
                 S->pending.pop_front();
             }
@@ -879,9 +1047,7 @@ static void server_loop(stntt_server *S) {
                 for (ServerRequest *r : round) {
                     r->done = true;
                     r->status = rc;
-                    S->used -= r->cost;
-                    S->cache[r->d_bytes].push_back(r->d_buf);
-                    r->d_buf = nullptr;
+                    server_cache_put(S, *r);
                 }
                 S->workers[w].inflight--;
                 for (ServerGroup &g : rd.groups) S->free_events.push_back(g.ev);
@@ -1361,6 +1527,51 @@ int stntt_auto_ntt_goldilocks(stntt_ctx *ctx, uint64_t *host_vec, size_t host_le
     });
 }
 
+static int check_steps_args(uint32_t log2n, uint32_t mod, int nsteps) {
+    if (log2n > 30) return fail(STNTT_EINVAL, "log2n %u too large", log2n);
+    if (mod < 2 || mod >= (1u << 31)) return fail(STNTT_EINVAL, "mod %u outside [2, 2^31)", mod);
+    if (nsteps > MAX_STAGE_STEPS) return fail(STNTT_EINVAL, "nsteps %d above %d", nsteps, MAX_STAGE_STEPS);
+    return STNTT_OK;
+}
+
+int stntt_stockham_steps_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t root,
+                             uint32_t mod, int nsteps, uint32_t flags, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~STNTT_SIGNED_INPUT) return fail(STNTT_EINVAL, "flags 0x%x (only STNTT_SIGNED_INPUT)", flags);
+    int rc = check_steps_args(log2n, mod, nsteps);
+    if (rc) return rc;
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    if ((rc = set_device(ctx))) return rc;
+    return stockham_steps<true>(ctx, d_data, log2n, batches, root, mod, nsteps, flags & STNTT_SIGNED_INPUT,
+                                stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_stockham_idx_u32(stntt_ctx *ctx, uint32_t *d_data, uint32_t log2n, uint64_t batches, uint32_t mod,
+                           int nsteps, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    int rc = check_steps_args(log2n, mod, nsteps);
+    if (rc) return rc;
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    if ((rc = set_device(ctx))) return rc;
+    return stockham_steps<false>(ctx, d_data, log2n, batches, 0, mod, nsteps, false,
+                                 stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int stntt_auto_stockham_steps_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                                  uint32_t root, uint32_t mod, int nsteps, float *kernel_ms) {
+    return auto_ntt_common<int32_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, int32_t *d) {
+        return stntt_stockham_steps_u32(ctx, (uint32_t *)d, L, batches, root, mod, nsteps, STNTT_SIGNED_INPUT,
+                                        nullptr);
+    });
+}
+
+int stntt_auto_stockham_idx_i32(stntt_ctx *ctx, int32_t *host_vec, size_t host_len, uint32_t n, uint32_t batches,
+                                uint32_t mod, int nsteps, float *kernel_ms) {
+    return auto_ntt_common<int32_t>(ctx, host_vec, host_len, n, batches, kernel_ms, [&](uint32_t L, int32_t *d) {
+        return stntt_stockham_idx_u32(ctx, (uint32_t *)d, L, batches, mod, nsteps, nullptr);
+    });
+}
+
 int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out) {
     if (!ctx || !out || (count && !descs)) return fail(STNTT_EINVAL, "null argument");
     if (count > 0xFFFFFFFFull) return fail(STNTT_EINVAL, "too many descriptors");
@@ -1639,6 +1850,7 @@ int stntt_server_stats_get(stntt_server *srv, stntt_server_stats *out) {
     *out = stntt_server_stats{};
     out->completed = srv->completed;
     out->peak_bytes = srv->peak_used;
+    out->held_bytes = srv->used + srv->cached;
     out->rounds = srv->rounds_launched;
     out->batched = srv->batched_requests;
     if (!srv->latencies.empty()) {
@@ -1661,7 +1873,7 @@ int stntt_server_reset_stats(stntt_server *srv) {
     std::lock_guard<std::mutex> g(srv->mu);
     srv->latencies.clear();
     srv->completed = 0;
-    srv->peak_used = srv->used;
+    srv->peak_used = srv->used + srv->cached;
     srv->rounds_launched = 0;
     srv->batched_requests = 0;
     srv->any = false;

@@ -307,10 +307,55 @@ def auto_ntt(args: NttArgs, ctx: Context | None = None) -> float:
 
 
 def ntt_stockham(vec: np.ndarray, size: int, root: int, mod: int, batches: int = 1,
-                 inverse: bool = False, ctx: Context | None = None) -> np.ndarray:
-    """nttStockham(vec, size, root, mod, batches) on the GPU, in place."""
-    auto_ntt(NttArgs(vec, size, batches, root, mod, inverse=inverse), ctx)
+                 inverse: bool = False, ctx: Context | None = None, nsteps: int | None = None) -> np.ndarray:
+    """nttStockham(vec, size, root, mod, batches, nsteps) (src/ntt/ntt_cpu.cpp:46-90)
+    on the GPU, in place.  nsteps=None runs the whole transform on the
+    multi-pass path; a value runs the reference's first nsteps radix-2
+    stages one by one (stntt_stockham_steps_u32), forward only."""
+    if nsteps is None:
+        auto_ntt(NttArgs(vec, size, batches, root, mod, inverse=inverse), ctx)
+        return vec
+    if inverse:
+        raise ValueError("nsteps applies to the forward stage loop only")
+    _stage_loop(vec, size, batches, root, mod, nsteps, True, ctx)
     return vec
+
+
+def ntt_stockham_idx(vec: np.ndarray, size: int, root: int, mod: int, batches: int = 1,
+                     nsteps: int | None = None, ctx: Context | None = None) -> np.ndarray:
+    """nttStockhamIdx(vec, size, root, mod, batches, nsteps) (src/ntt/ntt_cpu.cpp:92-127)
+    on the GPU, in place: the Stockham data movement without arithmetic
+    (with every stage, the bit-reversal permutation).  root is unused, as in
+    the reference; mod only selects the stage strides pow(2, i, mod)."""
+    _stage_loop(vec, size, batches, root, mod, (_log2_exact(size) if nsteps is None else nsteps), False, ctx)
+    return vec
+
+
+def _stage_loop(vec: np.ndarray, size: int, batches: int, root: int, mod: int, nsteps: int, arith: bool,
+                ctx: Context | None) -> None:
+    ctx = ctx or default_context()
+    if not isinstance(vec, np.ndarray) or not vec.flags.c_contiguous or vec.dtype not in (np.int32, np.uint32):
+        raise ValueError("vec must be a C-contiguous int32 or uint32 numpy array")
+    ms = ctypes.c_float()
+    root = int(root) % int(mod) if int(mod) > 0 else 0
+    if vec.dtype == np.int32:
+        if arith:
+            check(lib().stntt_auto_stockham_steps_i32(ctx.handle, vec.ctypes.data, vec.size, size, batches, root,
+                                                      mod, nsteps, ctypes.byref(ms)))
+        else:
+            check(lib().stntt_auto_stockham_idx_i32(ctx.handle, vec.ctypes.data, vec.size, size, batches, mod,
+                                                    nsteps, ctypes.byref(ms)))
+        return
+    L = _log2_exact(size)
+    if vec.size < size * batches:
+        raise IndexError("Host buffer is smaller than device buffer")
+    buf = ctx.upload(vec[: size * batches])
+    if arith:
+        check(lib().stntt_stockham_steps_u32(ctx.handle, buf.ptr, L, batches, root, mod, nsteps, 0, None))
+    else:
+        check(lib().stntt_stockham_idx_u32(ctx.handle, buf.ptr, L, batches, mod, nsteps, None))
+    buf.copy_to(vec[: size * batches])
+    buf.free()
 
 
 def ntt_goldilocks(vec: np.ndarray, size: int, batches: int = 1, inverse: bool = False,
@@ -495,6 +540,14 @@ def set_pass_split(ctx: Context, field: str, log2n: int, logr: list[int] | None)
     f = _FIELDS[field]
     arr = (ctypes.c_uint32 * max(len(logr or []), 1))(*(logr or []))
     check(lib().stntt_set_pass_split(ctx.handle, f, log2n, len(logr or []), arr))
+
+
+def restore_pass_split(ctx: Context, field: str, log2n: int) -> None:
+    """Put back the tuned split of (field, log2n) from the tuning table (or
+    the fewest-pass default where the table has none), undoing a
+    set_pass_split override."""
+    from .autotune import table_pass_split
+    set_pass_split(ctx, field, log2n, table_pass_split(field, log2n))
 
 
 def set_batch_split(ctx: Context, field: str, log2n: int, radix_logs: list[int] | None) -> None:

@@ -69,7 +69,7 @@ def test_u64_radix512_passes(ctx, split, inverse):
         assert np.array_equal(buf.copy_to(np.empty_like(x)), want), split
         buf.free()
     finally:
-        S.set_pass_split(ctx, "u64", L, None)
+        S.restore_pass_split(ctx, "u64", L)
 
 
 @pytest.mark.parametrize("split", [[9, 5, 6], [6, 5, 9]])
@@ -99,7 +99,7 @@ def test_u64_radix512_orders_and_polymul(ctx, split):
         da.free()
         db.free()
     finally:
-        S.set_pass_split(ctx, "u64", L, None)
+        S.restore_pass_split(ctx, "u64", L)
 
 
 @pytest.mark.parametrize("L", [12, 20, 24])

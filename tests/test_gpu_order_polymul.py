@@ -116,7 +116,7 @@ def test_orders_with_odd_radix_splits(ctx, field, L, split):
             ctx.ntt_u32(buf, L, 1, root, mod, inverse=True, bitrev_in=True)
         assert np.array_equal(buf.copy_to(np.empty_like(x)), x)
     finally:
-        S.set_pass_split(ctx, field, L, None)
+        S.restore_pass_split(ctx, field, L)
 
 
 @pytest.mark.parametrize("L", [1, 4, 10, 14, 17])

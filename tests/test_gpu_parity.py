@@ -298,7 +298,7 @@ def test_tuned_pass_splits_same_result(ctx, field, L, split):
             assert np.array_equal(run_u32(ctx, x, L, 1, root, mod), O.stockham_u32(x, n, root, mod))
             assert np.array_equal(run_u32(ctx, x, L, 1, root, mod, inverse=True), O.inverse_u32(x, n, root, mod))
     finally:
-        S.set_pass_split(ctx, field, L, None)
+        S.restore_pass_split(ctx, field, L)
 
 
 def test_pass_split_validation(ctx):

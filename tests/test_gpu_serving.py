@@ -94,6 +94,34 @@ def test_budget_admission_serializes_but_completes():
     srv.close()
 
 
+def test_size_classes_share_the_budget():
+    """Requests of many size classes through a small budget: idle cached
+    buffers count against it and are released (other classes first) when a
+    new class needs the room, so the bytes held never pass the budget and
+    every request completes exactly."""
+    room = 3 * (8 << 20)  # three 2^20 Goldilocks requests
+    budget = RESERVE + room
+    srv = S.Server(0, hbm_budget_bytes=budget, workers=1)
+    rng = np.random.default_rng(31)
+    sizes = [20, 19, 18, 17, 20, 16, 19, 15, 20, 18, 14, 20, 17, 19, 20, 13, 18, 20, 16, 20]
+    items = []
+    for wave in range(3):
+        ids = []
+        for L in sizes:
+            data, _, _ = make_request(srv, rng, "goldilocks", L, bool(wave % 2))
+            ids.append((L, data, data.array.copy(), srv.submit("goldilocks", L, data, inverse=bool(wave % 2))))
+        for L, data, x, rid in ids:
+            srv.wait(rid)
+            items.append((L, bool(wave % 2), data, x))
+        st = srv.stats()
+        assert st["peak_bytes"] <= room, (wave, st)
+        assert st["held_bytes"] <= room, (wave, st)
+    for L, inv, data, x in items:
+        assert np.array_equal(data.array, expected("goldilocks", L, inv, x, 0, 0)), (L, inv)
+        data.free()
+    srv.close()
+
+
 def test_concurrent_submitters():
     srv = S.Server(0, hbm_budget_bytes=2 * RESERVE + (1 << 30), workers=2)
     results, errors = [], []

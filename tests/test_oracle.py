@@ -64,6 +64,45 @@ def test_stockham_nsteps_matches_reference(ref_u32):
         assert np.array_equal(got.astype(np.int32), want), s
 
 
+@pytest.mark.skipif(not O.ref_available(), reason="the reference's CPU code is not built here")
+@pytest.mark.parametrize("L", range(0, 8))
+def test_stage_loop_past_log2n_matches_reference(L):
+    """nsteps beyond log2 n: the reference's strides J = pow(2, i, mod) and
+    its zeroed ping-pong buffer (src/ntt/ntt_cpu.cpp:52,64-84,98-122), with
+    moduli small enough that its int products do not wrap; the arithmetic
+    form and the index-only form, two rows each."""
+    R = O.ref()
+    n = 1 << L
+    for mod in (11, 13, 17, 97, 193, 257, 7681):
+        for nsteps in range(0, 14):
+            x = ((np.arange(2 * n, dtype=np.int64) * 7 + 3) % mod).astype(np.int32)
+            want = x.copy()
+            assert R.ref_ntt_stockham(want, n, 3, mod, 2, nsteps) == 0
+            got = O.stockham_u32(x.astype(np.uint32), n, 3, mod, 2, nsteps)
+            assert np.array_equal(got.astype(np.int32), want), (L, mod, nsteps)
+            xi = np.arange(2 * n, dtype=np.int32)
+            want = xi.copy()
+            assert R.ref_ntt_stockham_idx_mod(want, n, 2, mod, nsteps) == 0
+            got = np.concatenate([O.stockham_idx(xi[:n], n, nsteps, mod), O.stockham_idx(xi[n:], n, nsteps, mod)])
+            assert np.array_equal(got, want), (L, mod, nsteps)
+
+
+def test_stage_loop_past_log2n_closed_forms():
+    """Without the reference: n = 1 writes nothing, so the result is the
+    zeroed buffer after an odd number of stages and the input after an even
+    number; with mod = 17, pow(2, 8, 17) = 1 makes stage 8 of an 8-point
+    transform an ordinary J = 1 stage again."""
+    one = np.array([5], dtype=np.int32)
+    assert O.stockham_idx(one, 1, 3, 17)[0] == 0 and O.stockham_idx(one, 1, 2, 17)[0] == 5
+    assert O.stockham_u32(np.array([5], dtype=np.uint32), 1, 3, 17, 1, 1)[0] == 0
+    x = np.arange(8, dtype=np.int32)
+    nine = O.stockham_idx(x, 8, 9, 17)  # stages 0-2, then 3..7 (J > 4, nothing written), then J = 1
+    # after stages 0..7 the words are those of stage 2 or of the buffer two swaps back; stage 8 is
+    # a J = 1 stage applied to the buffer current after stage 7
+    eight = O.stockham_idx(x, 8, 8, 17)
+    assert np.array_equal(nine, O.stockham_idx(eight, 8, 1, 17))
+
+
 @pytest.mark.parametrize("L", [1, 4, 7, 10])
 def test_stockham_idx_is_bit_reversal(ref_u32, L):
     n = 1 << L
