@@ -110,6 +110,20 @@ int stntt_ntt_goldilocks(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint6
 int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
                             uint32_t flags, stntt_stream stream);
 
+/* stntt_ntt_goldilocks_ex that allocates nothing, for callers that cannot
+   call hipMalloc: code under HIP graph capture, tight serving loops,
+   harnesses that forbid allocation.  No counterpart in the reference.  The
+   twiddle tables and the multi-pass ping-pong live in one fixed device
+   array of the library (256 KiB of tables + 64 MiB of ping-pong per device,
+   held until the process ends); each call regenerates its tables there, so
+   a call -- or a replay of a captured call -- stands alone.  Limit: log2n
+   <= 28 and a schedule whose ping-pong fits 64 MiB (one transform up to
+   2^22, or 16 of 2^18; larger is STNTT_EINVAL).  Every call on a device
+   shares that device's array and keeps the last call's contents: calls
+   that overlap on one device (several streams or threads at once) race. */
+int stntt_ntt_goldilocks_noalloc(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                                 uint32_t flags, stntt_stream stream);
+
 /* Any odd prime p < 2^64 (STNTT_FIELD_U64): in place on d_data[batches][2^log2n]
    with a root of order exactly n (root^(n/2) = -1 mod p), e.g. from
    stntt_find_params(n, min_mod) with a 64-bit min_mod.  Every 64-bit input

@@ -49,6 +49,7 @@ SIGNATURES = [
     ("stntt_ntt_u32", _int, [_vp, _vp, _u32, _u64, _u32, _u32, _int, _u32, _vp]),
     ("stntt_ntt_goldilocks", _int, [_vp, _vp, _u32, _u64, _int, _vp]),
     ("stntt_ntt_goldilocks_ex", _int, [_vp, _vp, _u32, _u64, _int, _u32, _vp]),
+    ("stntt_ntt_goldilocks_noalloc", _int, [_vp, _vp, _u32, _u64, _int, _u32, _vp]),
     ("stntt_ntt_u64", _int, [_vp, _vp, _u32, _u64, _u64, _u64, _int, _u32, _vp]),
     ("stntt_twiddles_u64", _int, [_vp, _u32, _u64, _u64, _int, _vp]),
     ("stntt_pointwise_u64", _int, [_vp, _vp, _vp, _u64, _u64, _vp]),

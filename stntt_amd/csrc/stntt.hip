@@ -282,6 +282,39 @@ static int get_plan(stntt_ctx *ctx, int field, uint32_t log_n, uint64_t root, ui
     return STNTT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free Goldilocks transforms (stntt_ntt_goldilocks_noalloc): the
+// twiddle tables and the multi-pass ping-pong live in one fixed device
+// array, so a call allocates nothing.  Every call on a device shares that
+// device's copy and keeps the last call's contents; calls that overlap on
+// one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_TABLE_BYTES = 256 << 10;   // hi (2^14 words) + lo (up to 2^14 words)
+constexpr size_t NOALLOC_DATA_BYTES = 64ull << 20;  // ping-pong of the pass schedule
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_TABLE_BYTES + NOALLOC_DATA_BYTES];
+// The device address of `scratch` on each device (each holds its own copy);
+// in host code the array's name is not a device address.
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// once per device, with that device current (context creation)
+static int noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return STNTT_OK;  // the entry point reports it
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return STNTT_OK;
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+    return STNTT_OK;
+}
+
+static void *noalloc_ptr(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 template <class FP>
 __global__ void copy_kernel(const typename FP::T *in, typename FP::T *out, uint64_t count) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1210,6 +1243,7 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
     auto setup = [&]() -> int {
         HIP_TRY(hipSetDevice(device));
         HIP_TRY(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device));
+        if (int rc = noalloc_setup(device)) return rc;
         HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreate(&ctx->ev_start));
         HIP_TRY(hipEventCreate(&ctx->ev_stop));
@@ -1321,6 +1355,48 @@ int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, ui
     if ((rc = get_plan(ctx, STNTT_FIELD_GOLDILOCKS, log2n, 0, GL_P, direction, &plan))) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     return run_transform<FieldGL>(ctx, *plan, d_data, batches, flags, s);
+}
+
+int stntt_ntt_goldilocks_noalloc(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,
+                                 uint32_t flags, stntt_stream stream) {
+    if (!ctx) return fail(STNTT_EINVAL, "null context");
+    if (flags & ~(STNTT_BITREV_IN | STNTT_BITREV_OUT))
+        return fail(STNTT_EINVAL, "flags 0x%x (Goldilocks takes the order flags only)", flags);
+    if (log2n > 28) return fail(STNTT_EINVAL, "log2n %u above the allocation-free limit 28", log2n);
+    if (direction != STNTT_FORWARD && direction != STNTT_INVERSE)
+        return fail(STNTT_EINVAL, "bad direction %d", direction);
+    if (batches && !d_data) return fail(STNTT_EINVAL, "null data pointer");
+    if (batches == 0 || log2n == 0) return STNTT_OK;
+    if (batches >= (1ull << 31) || (batches << log2n) >> log2n != batches)
+        return fail(STNTT_EINVAL, "%llu transforms of 2^%u words is more than one call handles",
+                    (unsigned long long)batches, log2n);
+    const Schedule sch = transform_schedule<FieldGL>(ctx, STNTT_FIELD_GOLDILOCKS, log2n, batches);
+    const uint64_t need = schedule_scratch_bytes<FieldGL>(sch, batches << log2n);
+    if (need > NOALLOC_DATA_BYTES)
+        return fail(STNTT_EINVAL,
+                    "%llu transforms of 2^%u words need %llu bytes of scratch; the allocation-free path holds %llu",
+                    (unsigned long long)batches, log2n, (unsigned long long)need,
+                    (unsigned long long)NOALLOC_DATA_BYTES);
+    int rc;
+    if ((rc = set_device(ctx))) return rc;
+    void *base = noalloc_ptr(ctx->device);
+    if (!base) return fail(STNTT_EDEVICE, "no allocation-free scratch on device %d", ctx->device);
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    // the plan's factor tables, regenerated into the array by every call so
+    // that each call (and each replay of a captured one) stands alone
+    Plan plan{};
+    plan.key = PlanKey{STNTT_FIELD_GOLDILOCKS, direction, log2n, 0, GL_P};
+    plan.log_rmax = log2n < MAX_LOGR ? log2n : MAX_LOGR;
+    plan.d_tables = base;
+    const uint64_t n = 1ull << log2n, rmax = 1ull << plan.log_rmax, nlo = n >> plan.log_rmax;
+    uint64_t w = powmod(7, (GL_P - 1) >> log2n, GL_P);
+    if (direction == STNTT_INVERSE) w = invmod(w, GL_P);
+    plan.rt_gl.ninv = invmod(n % GL_P, GL_P);
+    hipLaunchKernelGGL(gen_tables_kernel<FieldGL>, dim3((uint32_t)((rmax + nlo + 255) / 256)), dim3(256), 0, s,
+                       (uint64_t *)base, (uint32_t)rmax, (uint32_t)nlo, w, powmod(w, nlo, GL_P), plan.rt_gl);
+    HIP_TRY(hipGetLastError());
+    return run_transform<FieldGL>(ctx, plan, d_data, batches, flags, s, nullptr, nullptr, nullptr,
+                                  (uint64_t *)((char *)base + NOALLOC_TABLE_BYTES));
 }
 
 int stntt_pointwise_u32(stntt_ctx *ctx, uint32_t *d_a, const uint32_t *d_b, uint64_t count, uint32_t mod,
