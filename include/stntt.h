@@ -257,9 +257,16 @@ int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t ns
    Every context starts with the splits tuned on MI355X (compiled into the
    library from stntt_amd/tuning/gfx950.json); this replaces one of them,
    and npasses = 0 removes it (the fewest-pass default then applies).  Each
-   pass covers 4..9 stages.  Applies to single transforms (batches == 1,
+   pass covers 4..12 stages (10..12: the large-radix kernel).  Applies to single transforms (batches == 1,
    what the autotuner times); batches keep the fewest-pass default. */
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr);
+/* stntt_set_pass_split with the tile shape too: passes of radix 2^10..2^12
+   run the large-radix kernel on tiles of R x 2^logc[i] words (the full
+   128 KiB tile down to a quarter of it; logc[i] = -1 picks the widest tile
+   that gives every CU two tiles); passes up to 2^9 take -1 (their tiles have
+   128-byte rows).  logc may be null (all -1).  Used by the autotuner. */
+int stntt_set_pass_shape(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr,
+                         const int32_t *logc);
 
 /* ---- serving: continuous batching under an HBM budget -------------------
    No counterpart in the reference, whose drivers run one synchronous

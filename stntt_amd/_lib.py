@@ -75,6 +75,7 @@ SIGNATURES = [
     ("stntt_batch_destroy", _int, [_vp]),
     ("stntt_set_batch_split", _int, [_vp, _int, _u32, _u32, _u32]),
     ("stntt_set_pass_split", _int, [_vp, _int, _u32, _u32, _c.POINTER(_u32)]),
+    ("stntt_set_pass_shape", _int, [_vp, _int, _u32, _u32, _c.POINTER(_u32), _c.POINTER(_c.c_int32)]),
     ("stntt_server_create", _int, [_int, _u64, _u32, _c.POINTER(_vp)]),
     ("stntt_server_destroy", _int, [_vp]),
     ("stntt_server_host_alloc", _int, [_vp, _sz, _c.POINTER(_vp)]),

@@ -40,7 +40,7 @@ def word_log2c(field: str) -> int:
 
 def apply_table(ctx, path: str = TABLE) -> int:
     """Install a tuning table into ctx; returns the number of entries applied."""
-    from .ntt import set_batch_split, set_pass_split
+    from .ntt import set_batch_split, set_pass_shape
     if not os.path.exists(path):
         return 0
     with open(path) as f:
@@ -48,7 +48,8 @@ def apply_table(ctx, path: str = TABLE) -> int:
     k = 0
     for field in PASS_FIELDS:
         for L, logr in table.get("pass_split", {}).get(field, {}).items():
-            set_pass_split(ctx, field, int(L), logr)
+            logc = table.get("pass_logc", {}).get(field, {}).get(L) or [-1] * len(logr)
+            set_pass_shape(ctx, field, int(L), logr, logc)
             k += 1
     for field in FIELDS:
         for L, logs in table.get("batch_split", {}).get(field, {}).items():
@@ -57,14 +58,37 @@ def apply_table(ctx, path: str = TABLE) -> int:
     return k
 
 
-def table_pass_split(field: str, L: int, path: str = TABLE) -> list[int] | None:
-    """The tuned stages-per-pass split of (field, 2^L) in the table, or None."""
+def table_pass_split(field: str, L: int, path: str = TABLE) -> tuple[list[int], list[int]] | None:
+    """The tuned (stages per pass, tile columns log2 per pass or -1) of
+    (field, 2^L) in the table, or None."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
         table = json.load(f)
     v = table.get("pass_split", {}).get(field, {}).get(str(L))
-    return list(v) if v else None
+    if not v:
+        return None
+    c = table.get("pass_logc", {}).get(field, {}).get(str(L)) or [-1] * len(v)
+    return list(v), list(c)
+
+
+BIG_LO, BIG_HI, BIG_TILE_LOG_WORDS = 10, 12, 14  # the large-radix kernel (csrc/ntt_big.hpp)
+
+
+def big_candidates(L: int) -> list[tuple[list[int], list[int]]]:
+    """Two large-radix passes (2^10..2^12 each) with every tile width the
+    kernel takes (the full 2^14-word tile down to a quarter) that fits n/R >=
+    C and, on the second pass, J >= C."""
+    out = []
+    for a in range(BIG_LO, BIG_HI + 1):
+        b = L - a
+        if not BIG_LO <= b <= BIG_HI:
+            continue
+        for ca in range(BIG_TILE_LOG_WORDS - a, BIG_TILE_LOG_WORDS - a - 3, -1):
+            for cb in range(BIG_TILE_LOG_WORDS - b, BIG_TILE_LOG_WORDS - b - 3, -1):
+                if L - a >= ca and L - b >= cb and a >= cb:
+                    out.append(([a, b], [ca, cb]))
+    return out
 
 
 def pass_candidates(L: int, logc: int, lo: int = 4, hi: int = 9) -> list[list[int]]:
@@ -141,7 +165,7 @@ def measure(cands, install, fn, sync, reps: int, rounds: int = 3, warm_s: float 
 def tune(logs_pass=range(8, 27), logs_batch=range(10, 19), verbose: bool = True, fields=PASS_FIELDS,
          parts=("pass", "batch")) -> dict:
     import stntt_amd as S
-    from .ntt import set_batch_split, set_pass_split
+    from .ntt import set_batch_split, set_pass_shape
     ctx = S.Context(0, tuned=False)
     rng = np.random.default_rng(1)
     table = {"device": "gfx950", "pass_split": {f: {} for f in PASS_FIELDS}, "batch_split": {f: {} for f in FIELDS},
@@ -172,12 +196,16 @@ def tune(logs_pass=range(8, 27), logs_batch=range(10, 19), verbose: bool = True,
                 fn = lambda: ctx.ntt_u64(buf, L, 1, root, mod)  # noqa: E731
             else:
                 fn = lambda: ctx.ntt_goldilocks(buf, L, 1)  # noqa: E731
-            med = measure(cands, lambda c: set_pass_split(ctx, field, L, c), fn, ctx.sync, 20)
+            # shapes: (stages per pass, tile columns per pass or -1)
+            shapes = [(c, [-1] * len(c)) for c in cands] + (big_candidates(L) if field != "u64" else [])
+            med = measure(shapes, lambda c: set_pass_shape(ctx, field, L, c[0], c[1]), fn, ctx.sync, 20)
             times = {k: round(v * 1e6, 2) for k, v in med.items()}
-            best = min(cands, key=lambda c: med[str(c)])
+            best = min(shapes, key=lambda c: med[str(c)])
             best_t = med[str(best)]
-            set_pass_split(ctx, field, L, best)
-            table["pass_split"][field][str(L)] = best
+            set_pass_shape(ctx, field, L, best[0], best[1])
+            table["pass_split"][field][str(L)] = best[0]
+            if any(x >= 0 for x in best[1]):
+                table.setdefault("pass_logc", {}).setdefault(field, {})[str(L)] = best[1]
             table["measured_us"]["pass"][field][str(L)] = times
             if verbose:
                 print(f"[tune] pass {field} 2^{L}: {best} {best_t * 1e6:.1f} us", flush=True)
@@ -228,6 +256,11 @@ def merge(old: dict, new: dict) -> dict:
     for kind in ("pass_split", "batch_split"):
         for field, entries in new.get(kind, {}).items():
             out.setdefault(kind, {}).setdefault(field, {}).update(entries)
+            if kind == "pass_split":  # a re-tuned size drops its old tile widths
+                for L in entries:
+                    out.get("pass_logc", {}).get(field, {}).pop(L, None)
+    for field, entries in new.get("pass_logc", {}).items():
+        out.setdefault("pass_logc", {}).setdefault(field, {}).update(entries)
     for kind, per_field in new.get("measured_us", {}).items():
         for field, entries in per_field.items():
             out.setdefault("measured_us", {}).setdefault(kind, {}).setdefault(field, {}).update(entries)
@@ -241,12 +274,14 @@ def main(argv=None) -> int:
     ap.add_argument("--fields", default=",".join(PASS_FIELDS),
                     help="comma list of u32, goldilocks, u64 (u64: pass splits only)")
     ap.add_argument("--parts", default="pass,batch", help="comma list of pass, batch")
+    ap.add_argument("--logs", default="8-26", help="log2 n range of the pass splits, e.g. 19-26")
     args = ap.parse_args(argv)
+    lo, hi = (int(v) for v in args.logs.split("-"))
     fields = tuple(f for f in args.fields.split(",") if f)
     parts = tuple(p for p in args.parts.split(",") if p)
     if any(f not in PASS_FIELDS for f in fields) or any(p not in ("pass", "batch") for p in parts):
         ap.error("unknown field or part")
-    table = tune(fields=fields, parts=parts)
+    table = tune(logs_pass=range(lo, hi + 1), fields=fields, parts=parts)
     if os.path.exists(args.out):
         with open(args.out) as f:
             table = merge(json.load(f), table)

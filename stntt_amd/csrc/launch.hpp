@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ntt_batch.hpp"
+#include "ntt_big.hpp"
 #include "ntt_pass.hpp"
 
 namespace stntt {
@@ -30,6 +31,21 @@ inline void launch_pass(int dir, bool single, bool first, bool perm, bool narrow
                         uint32_t grid, hipStream_t s) {
     if (dir == 0) launch_pass_dir<FP, 0>(single, first, perm, narrow, logr, a, grid, s);
     else launch_pass_dir<FP, 1>(single, first, perm, narrow, logr, a, grid, s);
+}
+
+// One pass of the large-radix kernel (ntt_big.hpp): radix 2^logr (10..12),
+// 2^logc columns per tile (at most big_full_logc, at least two fewer),
+// `ntiles` tiles over the whole batch, a persistent grid of at most
+// num_cus times the kernel's occupancy workgroups.
+template <class FP, int DIR>
+void launch_big_dir(bool first, int logr, int logc, const PassArgs<FP> &a, uint32_t ntiles, int num_cus,
+                    hipStream_t s);
+
+template <class FP>
+inline void launch_big(int dir, bool first, int logr, int logc, const PassArgs<FP> &a, uint32_t ntiles, int num_cus,
+                       hipStream_t s) {
+    if (dir == 0) launch_big_dir<FP, 0>(first, logr, logc, a, ntiles, num_cus, s);
+    else launch_big_dir<FP, 1>(first, logr, logc, a, ntiles, num_cus, s);
 }
 
 // The persistent descriptor-queue kernel (ntt_batch.hpp) and its counter

@@ -10,6 +10,7 @@
 
 #include <type_traits>
 
+#include "ntt_big.hpp"
 #include "ntt_pass.hpp"
 #include "ntt_tile.hpp"
 
@@ -19,6 +20,8 @@ constexpr int PASS_THREADS = 256;
 constexpr int MAX_LOGR = 14;        // largest pass radix; also the size of the hi twiddle factor
 constexpr int MIN_LOGR_TILE = 4;    // smallest radix of a tile pass
 constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile)
+constexpr int MIN_LOGR_BIG = 10;    // passes of radix 2^10..2^12 run the large-radix kernel
+constexpr int MAX_LOGR_BIG = 12;
 constexpr int NARROW_MAX_LOGR = 9;  // single-pass sizes with a one-transform, one-wave variant
 constexpr int NARROW_THREADS = 64;
 
@@ -45,7 +48,28 @@ struct Schedule {
     bool single = true;
     int npasses = 0;
     int logr[4] = {0, 0, 0, 0};
+    int logc[4] = {-1, -1, -1, -1};  // large-radix passes: tile columns (log2), -1 = big_logc_choice
 };
+
+// Columns (log2) of a large-radix pass over `batches` transforms of 2^L
+// words, or -1 when no tile shape fits: at most the full 128 KiB tile and at
+// least a quarter of it, with n / R >= C columns and, on a later pass,
+// J >= C so the columns of a tile share their pass twiddles.  `want` >= 0
+// asks for that width; otherwise the widest tile that still gives every CU
+// two tiles, else the narrowest that fits.
+template <class FP>
+inline int big_logc_choice(int L, int logr, int log_j, bool first, uint64_t batches, int num_cus, int want = -1) {
+    const int full = big_full_logc<FP>(logr);
+    auto fits = [&](int lc) { return lc >= full - 2 && lc <= full && L - logr >= lc && (first || log_j >= lc); };
+    if (want >= 0) return fits(want) ? want : -1;
+    int last = -1;
+    for (int lc = full; lc >= full - 2; --lc) {
+        if (!fits(lc)) continue;
+        last = lc;
+        if ((batches << (L - logr)) >> lc >= 2ull * (uint64_t)num_cus) return lc;
+    }
+    return last;
+}
 
 // smallest L a tile schedule covers: every pass needs n / R >= C columns and
 // every later pass J >= C
@@ -56,8 +80,11 @@ constexpr int min_tile_log() {
 
 // Single LDS pass when the transform is small or the batch alone fills the
 // GPU; otherwise 2-3 tile passes of at most 2^9 points each (2^24: 8+8+8).
+// Sizes whose default schedule is two large-radix passes (2^10..2^12 each).
+constexpr bool BIG_BY_DEFAULT = false;
+
 template <class FP>
-inline Schedule make_schedule(int L, uint64_t batches) {
+inline Schedule make_schedule(int L, uint64_t batches, bool allow_big = true) {
     Schedule s;
     const bool fits_one = L <= MAX_LOGR_SINGLE<FP>;
     if (L < min_tile_log<FP>() || (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
@@ -67,6 +94,12 @@ inline Schedule make_schedule(int L, uint64_t batches) {
         return s;
     }
     s.single = false;
+    if (BIG_BY_DEFAULT && allow_big && L >= 2 * MIN_LOGR_BIG && L <= 2 * MAX_LOGR_BIG) {
+        s.npasses = 2;
+        s.logr[0] = L / 2;
+        s.logr[1] = L - L / 2;
+        return s;
+    }
     int np = (L + MAX_LOGR_TILE - 1) / MAX_LOGR_TILE;
     if (np < 2) np = 2;
     s.npasses = np;

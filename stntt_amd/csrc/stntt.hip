@@ -142,8 +142,8 @@ struct stntt_ctx {
     int prof_count = 0;  // events recorded by the last profiled transform
     // autotuned sub-step splits of the batched kernel: (field, log2n) -> (nsub, radix_logs)
     std::map<std::pair<int, uint32_t>, std::pair<uint32_t, uint32_t>> batch_split;
-    // autotuned tile-pass splits: (field, log2n) -> stages per pass
-    std::map<std::pair<int, uint32_t>, std::vector<int>> pass_split;
+    // autotuned tile-pass splits: (field, log2n) -> (stages, tile columns log2 or -1) per pass
+    std::map<std::pair<int, uint32_t>, std::vector<std::pair<int, int>>> pass_split;
     int num_cus = 0;
 };
 
@@ -439,14 +439,26 @@ static int stockham_steps(stntt_ctx *ctx, uint32_t *d_data, uint32_t log_n, uint
 // default, or the context's tuned split for a single transform (tuned
 // splits were measured on one transform, i.e. latency; a batch is
 // throughput-bound and keeps the default).
+// Bit-reversed orders stage a tile through LDS on the first or last pass,
+// which only the 128-byte-row tile kernel does: schedules for them use no
+// large-radix pass.
 template <class FP>
-static Schedule transform_schedule(const stntt_ctx *ctx, int field, uint32_t L, uint64_t batches) {
-    Schedule sch = make_schedule<FP>((int)L, batches);
+static Schedule transform_schedule(const stntt_ctx *ctx, int field, uint32_t L, uint64_t batches,
+                                   uint32_t flags = 0) {
+    const bool allow_big = !(flags & (STNTT_BITREV_IN | STNTT_BITREV_OUT));
+    Schedule sch = make_schedule<FP>((int)L, batches, allow_big);
     if (!sch.single && batches == 1) {
         auto it = ctx->pass_split.find({field, L});
         if (it != ctx->pass_split.end()) {
-            sch.npasses = (int)it->second.size();
-            for (int i = 0; i < sch.npasses; ++i) sch.logr[i] = it->second[i];
+            bool big = false;
+            for (const auto &pr : it->second) big = big || pr.first >= MIN_LOGR_BIG;
+            if (allow_big || !big) {
+                sch.npasses = (int)it->second.size();
+                for (int i = 0; i < sch.npasses; ++i) {
+                    sch.logr[i] = it->second[i].first;
+                    sch.logc[i] = it->second[i].second;
+                }
+            }
         }
     }
     return sch;
@@ -482,13 +494,25 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         return fail(STNTT_EINVAL, "%llu transforms of 2^%u words is more than one call handles",
                     (unsigned long long)batches, L);
     if (L == 0) return STNTT_OK;  // zero stages: identity (src/ntt/ntt_cpu.cpp:64-84)
-    const Schedule sch = transform_schedule<FP>(ctx, plan.key.field, L, batches);
-    if (!sch.single)
+    const Schedule sch = transform_schedule<FP>(ctx, plan.key.field, L, batches, flags);
+    // tile columns (log2) of each pass
+    int logc[4] = {0, 0, 0, 0};
+    if (!sch.single) {
+        uint32_t lj = 0;
         for (int p = 0; p < sch.npasses; ++p) {
-            const uint64_t tiles = batches * ((((1ull << L) >> sch.logr[p])) >> tile_logc<FP>());
+            if (sch.logr[p] >= MIN_LOGR_BIG) {
+                logc[p] = big_logc_choice<FP>((int)L, sch.logr[p], (int)lj, p == 0, batches, ctx->num_cus, sch.logc[p]);
+                if (logc[p] < 0)
+                    return fail(STNTT_EINVAL, "no tile shape for a radix-2^%d pass of 2^%u words", sch.logr[p], L);
+            } else {
+                logc[p] = tile_logc<FP>();
+            }
+            const uint64_t tiles = batches * ((((1ull << L) >> sch.logr[p])) >> logc[p]);
             if (tiles >= (1ull << 31))
                 return fail(STNTT_EINVAL, "%llu tiles exceed one launch", (unsigned long long)tiles);
+            lj += sch.logr[p];
         }
+    }
     const bool inv = plan.key.dir == STNTT_INVERSE;
     const bool rev_in = flags & STNTT_BITREV_IN, rev_out = flags & STNTT_BITREV_OUT;
     if (mul_in && !sch.single && !rev_in)
@@ -555,10 +579,13 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
                 grid = (uint32_t)batches;
             }
         } else {
-            a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> tile_logc<FP>());
+            a.tiles_per_transform = (uint32_t)(((1ull << L) >> logr) >> logc[p]);
             grid = (uint32_t)(batches * a.tiles_per_transform);  // checked before the first launch
         }
-        launch_pass<FP>(plan.key.dir, sch.single, first, perm, narrow, logr, a, grid, s);
+        if (!sch.single && logr >= MIN_LOGR_BIG)
+            launch_big<FP>(plan.key.dir, first, logr, logc[p], a, grid, ctx->num_cus, s);
+        else
+            launch_pass<FP>(plan.key.dir, sch.single, first, perm, narrow, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
         src = dst;
@@ -1252,7 +1279,7 @@ int stntt_ctx_create(int device, stntt_ctx **out) {
         // the splits tuned on this architecture (tuned_gfx950.hpp); callers
         // may replace any of them with stntt_set_pass_split / _batch_split
         for (const TunedPassSplit &t : TUNED_PASS_SPLITS)
-            if (int rc = stntt_set_pass_split(ctx.get(), t.field, t.log2n, t.npasses, t.logr)) return rc;
+            if (int rc = stntt_set_pass_shape(ctx.get(), t.field, t.log2n, t.npasses, t.logr, t.logc)) return rc;
         for (const TunedBatchSplit &t : TUNED_BATCH_SPLITS)
             if (int rc = stntt_set_batch_split(ctx.get(), t.field, t.log2n, t.nsub, t.radix_logs)) return rc;
         return STNTT_OK;
@@ -1957,6 +1984,11 @@ int stntt_server_reset_stats(stntt_server *srv) {
 }
 
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr) {
+    return stntt_set_pass_shape(ctx, field, log2n, npasses, logr, nullptr);
+}
+
+int stntt_set_pass_shape(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr,
+                         const int32_t *logc) {
     if (!ctx) return fail(STNTT_EINVAL, "null context");
     if (field != STNTT_FIELD_U32 && field != STNTT_FIELD_GOLDILOCKS && field != STNTT_FIELD_U64)
         return fail(STNTT_EINVAL, "bad field");
@@ -1965,20 +1997,33 @@ int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npa
         return STNTT_OK;
     }
     if (npasses < 2 || npasses > 4 || !logr) return fail(STNTT_EINVAL, "pass count %u outside [2, 4]", npasses);
-    const int logc = field == STNTT_FIELD_U32 ? tile_logc<FieldU32>() : tile_logc<FieldGL>();
+    const int tlogc = field == STNTT_FIELD_U32 ? tile_logc<FieldU32>() : tile_logc<FieldGL>();
     uint32_t sum = 0;
-    std::vector<int> v;
+    std::vector<std::pair<int, int>> v;
     for (uint32_t i = 0; i < npasses; ++i) {
-        if (logr[i] < (uint32_t)MIN_LOGR_TILE || logr[i] > (uint32_t)MAX_LOGR_TILE)
-            return fail(STNTT_EINVAL, "pass radix 2^%u outside [2^%d, 2^%d]", logr[i], MIN_LOGR_TILE, MAX_LOGR_TILE);
-        // a pass needs n / R >= C columns, and a later pass J >= C
-        if (i > 0 && sum < (uint32_t)logc) return fail(STNTT_EINVAL, "pass %u starts with J < %d columns", i, 1 << logc);
-        sum += logr[i];
-        v.push_back((int)logr[i]);
+        const uint32_t r = logr[i];
+        const int want = logc ? logc[i] : -1;
+        if (r < (uint32_t)MIN_LOGR_TILE || r > (uint32_t)MAX_LOGR_BIG)
+            return fail(STNTT_EINVAL, "pass radix 2^%u outside [2^%d, 2^%d]", r, MIN_LOGR_TILE, MAX_LOGR_BIG);
+        if (r <= (uint32_t)MAX_LOGR_TILE) {
+            // a tile pass needs n / R >= C columns, and a later pass J >= C
+            if (want >= 0 && want != tlogc)
+                return fail(STNTT_EINVAL, "pass %u: a radix-2^%u tile pass has 2^%d columns", i, r, tlogc);
+            if (i > 0 && sum < (uint32_t)tlogc) return fail(STNTT_EINVAL, "pass %u starts with J < %d columns", i, 1 << tlogc);
+            if (log2n < r + (uint32_t)tlogc) return fail(STNTT_EINVAL, "pass %u has fewer than %d columns", i, 1 << tlogc);
+        } else {
+            // a large-radix pass: some tile width from the full 128 KiB down to a quarter must fit
+            int lc = -1;
+            if (field == STNTT_FIELD_U32)
+                lc = big_logc_choice<FieldU32>((int)log2n, (int)r, (int)sum, i == 0, 1, ctx->num_cus, want);
+            else
+                lc = big_logc_choice<FieldGL>((int)log2n, (int)r, (int)sum, i == 0, 1, ctx->num_cus, want);
+            if (lc < 0) return fail(STNTT_EINVAL, "pass %u: no tile shape for radix 2^%u", i, r);
+        }
+        sum += r;
+        v.push_back({(int)r, want});
     }
     if (sum != log2n) return fail(STNTT_EINVAL, "pass radices cover 2^%u, not 2^%u", sum, log2n);
-    for (uint32_t i = 0; i < npasses; ++i)
-        if (log2n - logr[i] < (uint32_t)logc) return fail(STNTT_EINVAL, "pass %u has fewer than %d columns", i, 1 << logc);
     ctx->pass_split[{field, log2n}] = v;
     return STNTT_OK;
 }

@@ -542,12 +542,25 @@ def set_pass_split(ctx: Context, field: str, log2n: int, logr: list[int] | None)
     check(lib().stntt_set_pass_split(ctx.handle, f, log2n, len(logr or []), arr))
 
 
+def set_pass_shape(ctx: Context, field: str, log2n: int, logr: list[int], logc: list[int]) -> None:
+    """A pass split with the tile columns (log2) of each pass; -1 = the
+    library's choice.  Passes of radix 2^10..2^12 run the large-radix kernel."""
+    f = _FIELDS[field]
+    r = (ctypes.c_uint32 * len(logr))(*logr)
+    c = (ctypes.c_int32 * len(logc))(*logc)
+    check(lib().stntt_set_pass_shape(ctx.handle, f, log2n, len(logr), r, c))
+
+
 def restore_pass_split(ctx: Context, field: str, log2n: int) -> None:
     """Put back the tuned split of (field, log2n) from the tuning table (or
     the fewest-pass default where the table has none), undoing a
     set_pass_split override."""
     from .autotune import table_pass_split
-    set_pass_split(ctx, field, log2n, table_pass_split(field, log2n))
+    t = table_pass_split(field, log2n)
+    if t is None:
+        set_pass_split(ctx, field, log2n, None)
+    else:
+        set_pass_shape(ctx, field, log2n, t[0], t[1])
 
 
 def set_batch_split(ctx: Context, field: str, log2n: int, radix_logs: list[int] | None) -> None:

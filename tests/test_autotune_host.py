@@ -56,24 +56,40 @@ def test_compiled_table_matches_json():
     for field, entries in table["pass_split"].items():
         for L, split in entries.items():
             row = ", ".join(map(str, list(split) + [0] * (4 - len(split))))
-            assert f"{{{G.FIELD_IDS[field]}, {L}, {len(split)}, {{{row}}}}}" in text, (field, L)
+            logc = table.get("pass_logc", {}).get(field, {}).get(L) or [-1] * len(split)
+            crow = ", ".join(map(str, list(logc) + [-1] * (4 - len(logc))))
+            assert f"{{{G.FIELD_IDS[field]}, {L}, {len(split)}, {{{row}}}, {{{crow}}}}}" in text, (field, L)
 
 
 def test_table_checks_reject_what_the_library_rejects():
     import pytest
     for field, L, split in (("goldilocks", 24, [8, 8, 8]), ("u32", 24, [9, 7, 8]), ("goldilocks", 13, [9, 4])):
         G.check_pass_split(field, L, split)
-    for field, L, split in (("goldilocks", 24, [10, 7, 7]),  # radix above 2^9
-                            ("u32", 14, [4, 10]),            # radix above 2^9
+    # large-radix passes (2^10..2^12) with the tile widths the kernel takes
+    G.check_pass_split("goldilocks", 24, [12, 12], [2, 1])
+    G.check_pass_split("u32", 22, [11, 11], [-1, -1])
+    G.check_pass_split("goldilocks", 24, [10, 7, 7])
+    for field, L, split in (("goldilocks", 26, [13, 13]),    # radix above 2^12
                             ("u32", 12, [4, 8]),             # second pass with J = 2^4 < 32 columns
                             ("u32", 13, [9, 4]),             # n / R = 2^4 < 32 columns
                             ("u32", 24, [8, 8, 7])):         # covers 2^23
         with pytest.raises(AssertionError):
             G.check_pass_split(field, L, split)
     with pytest.raises(AssertionError):
+        G.check_pass_split("goldilocks", 24, [12, 12], [3, -1])   # wider than a 2^14-word tile
+    with pytest.raises(AssertionError):
+        G.check_pass_split("goldilocks", 24, [12, 12], [-1, 4])   # a tile pass width on a large-radix pass
+    with pytest.raises(AssertionError):
         G.check_batch_split("u32", 12, [5, 4, 3])  # sub-step radix above 2^4
     with pytest.raises(AssertionError):
         G.check_batch_split("u64", 12, [4, 4, 4])  # no batched 64-bit prime field
+
+
+def test_big_candidates_fit_the_kernel():
+    for L in range(20, 25):
+        for split, logc in A.big_candidates(L):
+            G.check_pass_split("goldilocks", L, split, logc)
+    assert ([12, 12], [2, 2]) in A.big_candidates(24) and not A.big_candidates(19)
 
 
 def test_measure_alternates_order_and_takes_medians():

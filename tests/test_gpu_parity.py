@@ -305,6 +305,6 @@ def test_pass_split_validation(ctx):
     with pytest.raises(ValueError):
         S.set_pass_split(ctx, "goldilocks", 24, [8, 8, 7])     # covers 23 stages
     with pytest.raises(ValueError):
-        S.set_pass_split(ctx, "goldilocks", 24, [12, 12])      # pass above 2^9
+        S.set_pass_split(ctx, "goldilocks", 26, [13, 13])      # pass above 2^12
     with pytest.raises(ValueError):
         S.set_pass_split(ctx, "u32", 12, [4, 4, 4])            # J = 16 < 32 columns
