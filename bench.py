@@ -16,7 +16,15 @@ peak.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 
 cpu_baseline: the oracle's restatement of the reference's CPU Stockham
 (src/ntt/ntt_cpu.cpp:46-90 with exact Goldilocks arithmetic), single
-thread, on a bounded sample of the same workload.
+thread, on a bounded sample of the same workload; the host's core count is
+reported beside it, and the batched and serving workloads also get an
+all-cores CPU figure (the same restatement on a thread pool over a bounded
+sample of their transforms).
+
+Clock state: the GPU clock ramps up over the first ~50 ms of load and drops
+when the host leaves it idle, so before the W warm-up steps the headline
+workload runs for --settle-s seconds (default 0.1 s, untimed); the line
+says so in "clock".
 """
 from __future__ import annotations
 
@@ -57,16 +65,25 @@ def parse():
                     help="also run a plain device copy of the same buffer (counter calibration)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--legs", default="n20,u32large,orders,field64,batched,serving",
+    ap.add_argument("--legs", default="n20,gllarge,u32large,orders,field64,batched,serving",
                     help="comma-separated extra legs to run after the headline")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched leg without the per-transform plain comparison")
     ap.add_argument("--serve-count", type=int, default=400)
+    ap.add_argument("--settle-s", type=float, default=0.1,
+                    help="seconds of untimed transforms before the warm-up steps (GPU clock ramp)")
     return ap.parse_args()
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
 
 
 def cpu_baseline(log2n: int, budget_s: float) -> dict:
@@ -81,9 +98,44 @@ def cpu_baseline(log2n: int, budget_s: float) -> dict:
         O.stockham_gl(x, log2n)
         times.append(time.perf_counter() - t0)
     med = statistics.median(times)
-    return {"value": n / med / 1e9, "unit": "GElem/s", "cores": 1, "kind": "port",
+    return {"value": n / med / 1e9, "unit": "GElem/s", "cores": 1, "host_cores": host_cores(), "kind": "port",
             "sample": f"{len(times)} whole Goldilocks 2^{log2n} forward transforms, median "
                       f"{med * 1e3:.0f} ms each, single thread (oracle/ntt_oracle.c)"}
+
+
+def cpu_all_cores(specs, budget_s: float, label: str) -> dict:
+    """The oracle on a thread pool of every host core (the C oracle releases
+    the GIL) over the first transforms of a workload until about budget_s:
+    transforms/s and elements/s of all cores."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    cores = host_cores()
+    params = {sp.log2n: O.find_params(1 << sp.log2n, 11) for sp in specs if sp.field == "u32"}
+
+    def one(i_sp):
+        i, sp = i_sp
+        n = 1 << sp.log2n
+        rng = np.random.default_rng(1000 + i)  # one generator per task: threads share none
+        if sp.field == "goldilocks":
+            x = rng.integers(0, P, size=n, dtype=np.uint64)
+            (O.inverse_gl if sp.inverse else O.stockham_gl)(x, sp.log2n)
+        else:
+            r, m = params[sp.log2n]
+            x = rng.integers(0, m, size=n, dtype=np.uint32)
+            (O.inverse_u32 if sp.inverse else O.stockham_u32)(x, n, r, m)
+        return n
+
+    done, elems = 0, 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        while done < len(specs) and time.perf_counter() - t0 < budget_s:
+            chunk = list(enumerate(specs))[done:done + 4 * cores]
+            elems += sum(ex.map(one, chunk))
+            done += len(chunk)
+    dt = time.perf_counter() - t0
+    return {"transforms_s": round(done / dt, 1), "gelem_s": round(elems / dt / 1e9, 4), "cores": cores,
+            "kind": "port", "sample": f"first {done} transforms of the {label} ({elems} elements) in "
+                                      f"{dt:.1f} s on {cores} threads (oracle/ntt_oracle.c)"}
 
 
 def bench_n20(S, ctx, steps: int) -> dict:
@@ -227,6 +279,34 @@ def bench_u32_large(S, ctx, steps: int) -> dict:
         ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
         out[f"2^{L}"] = {"mod": mod, "forward_us": round(fw * 1e6, 2), "inverse_us": round(iv * 1e6, 2),
                          "gelem_s": round(n / fw / 1e9, 2), "roundtrip_exact": bool(ok)}
+        buf.free()
+    return out
+
+
+def bench_goldilocks_large(S, ctx, steps: int) -> dict:
+    """Goldilocks 2^24 inverse and 2^26 forward and inverse, device-resident:
+    time per transform, GElem/s and the fraction of the 8 TB/s HBM roofline
+    against the algorithmic 2 n w bytes; round trips checked."""
+    out = {}
+    for L in (24, 26):
+        n = 1 << L
+        x = np.random.default_rng(100 + L).integers(0, P, size=n, dtype=np.uint64)
+        buf = ctx.upload(x)
+        fwd = lambda: ctx.ntt_goldilocks(buf, L, 1)  # noqa: E731
+        inv = lambda: ctx.ntt_goldilocks(buf, L, 1, inverse=True)  # noqa: E731
+        fw, nf = _time_steps(ctx, fwd, steps)
+        iv, ni = _time_steps(ctx, inv, steps)
+        for _ in range(nf - ni):
+            inv()
+        for _ in range(ni - nf):
+            fwd()
+        ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
+        alg = 2 * n * 8
+        out[f"2^{L}"] = {"forward_us": round(fw * 1e6, 2), "inverse_us": round(iv * 1e6, 2),
+                         "forward_gelem_s": round(n / fw / 1e9, 2), "inverse_gelem_s": round(n / iv / 1e9, 2),
+                         "forward_hbm_frac": round(alg / fw / 1e9 / HBM_PEAK_GBS, 4),
+                         "inverse_hbm_frac": round(alg / iv / 1e9 / HBM_PEAK_GBS, 4),
+                         "roundtrip_exact": bool(ok)}
         buf.free()
     return out
 
@@ -381,7 +461,12 @@ def main() -> int:
     ctx = S.Context(0)
     x = np.random.default_rng(42).integers(0, P, size=n, dtype=np.uint64)
     buf = ctx.upload(x)
-    log(f"[bench] goldilocks 2^{L}: warmup {args.warmup}")
+    log(f"[bench] goldilocks 2^{L}: clock settle {args.settle_s} s, warmup {args.warmup}")
+    t_end = time.perf_counter() + args.settle_s
+    while time.perf_counter() < t_end:
+        for _ in range(10):
+            ctx.ntt_goldilocks(buf, L, 1)
+        ctx.sync()
     for _ in range(args.warmup):
         ctx.ntt_goldilocks(buf, L, 1)
     ctx.sync()
@@ -447,7 +532,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"pass {dom} of {npass}", "kernel_ms": round(mean_pass[dom], 5)},
-        "hbm_frac_transform": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        # the transform as a whole: 2 n w bytes over the whole step
+        "hbm_frac_transform": round(alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_frac_transform_kernels": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "clock": f"{args.settle_s} s of untimed transforms, then {args.warmup} warm-up steps",
         # the bound that actually limits this kernel: VALU issue.  Wave64
         # instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/) over
         # the live mean duration, against 1024 SIMDs x 2.4 GHz / 4 cycles.
@@ -465,6 +553,9 @@ def main() -> int:
     if "orders" in legs:
         log("[bench] element orders and polynomial multiply")
         out["orders_polymul"] = bench_orders_polymul(S, ctx, 20)
+    if "gllarge" in legs:
+        log("[bench] Goldilocks 2^24 inverse, 2^26 forward and inverse")
+        out["goldilocks_large"] = bench_goldilocks_large(S, ctx, 20)
     if "u32large" in legs:
         log("[bench] 32-bit field at 2^24 and 2^26")
         out["u32_large"] = bench_u32_large(S, ctx, 20)
@@ -480,6 +571,15 @@ def main() -> int:
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)
+        if "batched" in legs or "serving" in legs:
+            from stntt_amd.workloads import mixed_batch, poisson_arrivals
+            log("[bench] cpu, all cores: batched and serving workloads")
+            out["cpu_all_cores"] = {
+                "batched_mixed": cpu_all_cores(mixed_batch(seed=42, count=args.batch_count, lo=10, hi=18),
+                                               args.cpu_seconds / 2, "batched mixed workload"),
+                "serving": cpu_all_cores([sp for _, sp in poisson_arrivals(seed=17, count=args.serve_count,
+                                                                           rate_per_s=1.0)],
+                                         args.cpu_seconds / 2, "serving trace")}
     buf.free()
     ctx.close()
     print(json.dumps(out), flush=True)
