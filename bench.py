@@ -425,21 +425,33 @@ def bench_serving(S, count: int, workers: int) -> dict:
     out = {"requests": count, "workers": workers, "budget_bytes": 64 << 30,
            "plain_sequential_capacity_tps": round(seq_capacity, 1), "rates": []}
     replay(4.0 * seq_capacity, True)  # warm the server's device-buffer cache and plans
-    for frac in (0.8, 2.0):
+    for frac in (0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0):
         rate = frac * seq_capacity
-        lat_plain, span_plain = replay(rate, False)
         srv.reset_stats()
         _, span_srv = replay(rate, True)
         st = srv.stats()
-        lp = np.array(lat_plain)
-        out["rates"].append({
-            "offered_tps": round(rate, 1),
-            "server": {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                       "transforms_s": round(count / span_srv, 1), "rounds": st["rounds"],
-                       "batched_requests": st["batched"], "peak_bytes": st["peak_bytes"]},
-            "plain_sequential": {"p50_ms": round(float(np.percentile(lp, 50)), 3),
-                                 "p99_ms": round(float(np.percentile(lp, 99)), 3),
-                                 "transforms_s": round(count / span_plain, 1)}})
+        row = {"offered_x_sequential": frac, "offered_tps": round(rate, 1),
+               "server": {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                          "transforms_s": round(count / span_srv, 1), "rounds": st["rounds"],
+                          "batched_requests": st["batched"], "grouped_requests": st["grouped"],
+                          "peak_bytes": st["peak_bytes"]}}
+        if frac in (0.8, 2.0):
+            lat_plain, span_plain = replay(rate, False)
+            lp = np.array(lat_plain)
+            row["plain_sequential"] = {"p50_ms": round(float(np.percentile(lp, 50)), 3),
+                                       "p99_ms": round(float(np.percentile(lp, 99)), 3),
+                                       "transforms_s": round(count / span_plain, 1)}
+        out["rates"].append(row)
+    # saturation: the most the server completed at any offered rate, and the
+    # highest offered rate it kept up with (completed >= 95 % of offered)
+    out["saturation_tps"] = max(r["server"]["transforms_s"] for r in out["rates"])
+    # host<->device bytes per request (upload + download of n words of 8 B):
+    # the rate the copies ran at, at saturation
+    mean_bytes = float(np.mean([2 * (8 << sp.log2n) for _, sp in trace0]))
+    out["pcie_bytes_per_request"] = round(mean_bytes)
+    out["saturation_pcie_gbs"] = round(out["saturation_tps"] * mean_bytes / 1e9, 1)
+    kept = [r["offered_tps"] for r in out["rates"] if r["server"]["transforms_s"] >= 0.95 * r["offered_tps"]]
+    out["max_sustained_offered_tps"] = max(kept) if kept else None
     dev.free()
     ctx.close()
     for a in bufs:

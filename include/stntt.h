@@ -293,6 +293,7 @@ typedef struct stntt_server_stats {
     uint64_t rounds;                        /* admission rounds launched */
     uint64_t batched;                       /* requests run through the batch kernel */
     uint64_t held_bytes;                    /* request bytes held now (admitted + cached) */
+    uint64_t grouped;                       /* requests run in a grouped multi-pass run of one shape */
 } stntt_server_stats;
 
 typedef struct stntt_server stntt_server;

@@ -116,7 +116,7 @@ class ServerStats(_c.Structure):
     """stntt_server_stats (include/stntt.h)"""
     _fields_ = [("completed", _u64), ("p50_ms", _c.c_double), ("p99_ms", _c.c_double), ("mean_ms", _c.c_double),
                 ("max_ms", _c.c_double), ("span_s", _c.c_double), ("peak_bytes", _u64), ("rounds", _u64),
-                ("batched", _u64), ("held_bytes", _u64)]
+                ("batched", _u64), ("held_bytes", _u64), ("grouped", _u64)]
 
 
 class NativeLibraryMissing(RuntimeError):

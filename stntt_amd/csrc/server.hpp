@@ -14,13 +14,15 @@
 //      admission order (one transfer at a time at full PCIe bandwidth, not
 //      shared among workers); a free worker stream waits for them and runs
 //      the round's small transforms (log2 n <= SERVER_BATCH_MAX_LOG) as one
-//      persistent descriptor-queue launch, then each larger one through the
-//      multi-pass path; the download stream takes each result back as soon
-//      as its own kernel is done.  Uploads, kernels and downloads of
-//      different rounds overlap;
+//      persistent descriptor-queue launch, then the larger ones through the
+//      multi-pass path, requests of one shape (field, size, direction,
+//      parameters) together as one grouped run over pointer tables, as many
+//      at a time as the worker's ping-pong scratch holds; the download
+//      stream takes each group's results back as soon as its kernels are
+//      done.  Uploads, kernels and downloads of different rounds overlap;
 //   3. retires each group of a round (the batched small requests, or one
-//      large request) when its download event has fired, moves its buffers
-//      to the cache and wakes waiters.
+//      grouped run of large requests) when its download event has fired,
+//      moves its buffers to the cache and wakes waiters.
 // The budget covers every device byte held for requests: admitted buffers
 // plus cached idle ones.  A request of a size class with no cached buffer
 // first releases cached buffers of other classes until its allocation fits;
@@ -71,6 +73,8 @@ struct ServerWorker {
     BatchTask *d_tasks = nullptr;
     uint32_t *d_counter = nullptr;
     uint64_t *d_scratch = nullptr;  // SERVER_BATCH_GRID slices of 2^SERVER_BATCH_MAX_LOG words
+    void **h_tab = nullptr;         // pinned staging of the round's grouped buffer pointers
+    void **d_tab = nullptr;         // their device copy (in and out tables of grouped runs)
     int inflight = 0;
 };
 
@@ -108,7 +112,7 @@ struct stntt_server {
     uint64_t completed = 0;
     uint64_t cached = 0;     // bytes of idle buffers in `cache` (held, charged to the budget)
     uint64_t peak_used = 0;  // peak of used + cached
-    uint64_t rounds_launched = 0, batched_requests = 0;
+    uint64_t rounds_launched = 0, batched_requests = 0, grouped_requests = 0;
     bool stop = false;
     std::thread thread;
     stntt::Clock::time_point t_first{}, t_last{};

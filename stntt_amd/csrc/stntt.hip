@@ -936,6 +936,20 @@ static hipEvent_t server_event(stntt_server *S) {
     return e;
 }
 
+// Requests of one shape (2^L words each) that one grouped run can take out
+// of `want`: as many as the worker's ping-pong scratch holds for the
+// multi-pass schedule of that many transforms (at least one).
+template <class FP>
+static uint64_t server_chunk(const stntt_ctx *ctx, int field, uint32_t L, uint64_t want) {
+    uint64_t c = std::min<uint64_t>(want, SERVER_MAX_ROUND);
+    while (c > 1) {
+        const uint64_t need = schedule_scratch_bytes<FP>(transform_schedule<FP>(ctx, field, L, c), c << L);
+        if (need <= ctx->scratch_bytes) break;
+        c = std::max<uint64_t>(1, std::min<uint64_t>(c - 1, ctx->scratch_bytes / (need / c)));
+    }
+    return c;
+}
+
 // Called by the server thread only (it alone touches the event pool).
 static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest *> &round, ServerRound &rd) {
     ServerWorker &W = S->workers[w];
@@ -1012,17 +1026,67 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
         int rc = download(small, reqs);
         if (rc) return rc;
     }
-    // 3. each large transform, its download right behind it
+    // 3. the larger transforms, grouped by shape (field, size, direction,
+    //    parameters, signed input) in order of first arrival.  A group runs
+    //    the multi-pass path once over pointer tables (in place), in chunks
+    //    that the worker's ping-pong scratch holds; a lone request runs the
+    //    tuned single-transform schedule.  Each chunk's download follows its
+    //    last pass.
+    using ShapeKey = std::tuple<int, int, uint32_t, uint32_t, uint64_t, uint64_t>;
+    std::map<ShapeKey, std::vector<ServerRequest *>> shapes;
+    std::vector<ShapeKey> order;
     for (ServerRequest *r : round) {
         const stntt_request &q = r->req;
         if (q.log2n <= SERVER_BATCH_MAX_LOG) continue;
-        int rc = q.field == STNTT_FIELD_GOLDILOCKS
-                     ? stntt_ntt_goldilocks(W.ctx, (uint64_t *)r->d_buf, q.log2n, 1, q.direction, s)
-                     : stntt_ntt_u32(W.ctx, (uint32_t *)r->d_buf, q.log2n, 1, q.root, q.mod, q.direction, q.flags, s);
-        if (rc) return rc;
-        ServerGroup g;
-        g.ids.push_back(r->id);
-        if ((rc = download(g, {r}))) return rc;
+        const bool u32 = q.field == STNTT_FIELD_U32;
+        ShapeKey k{(int)q.field, (int)q.direction, q.log2n, u32 ? (q.flags & STNTT_SIGNED_INPUT) : 0u,
+                   u32 ? q.root % q.mod : 0, u32 ? q.mod : GL_P};
+        auto &v = shapes[k];
+        if (v.empty()) order.push_back(k);
+        v.push_back(r);
+    }
+    uint32_t ntab = 0;  // pointer-table entries this round has used
+    for (const ShapeKey &k : order) {
+        const std::vector<ServerRequest *> &v = shapes[k];
+        const stntt_request &q0 = v.front()->req;
+        const bool gl = q0.field == STNTT_FIELD_GOLDILOCKS;
+        for (size_t i = 0; i < v.size();) {
+            const uint64_t left = v.size() - i;
+            const uint64_t chunk = gl ? server_chunk<FieldGL>(W.ctx, STNTT_FIELD_GOLDILOCKS, q0.log2n, left)
+                                      : server_chunk<FieldU32>(W.ctx, STNTT_FIELD_U32, q0.log2n, left);
+            std::vector<ServerRequest *> part(v.begin() + i, v.begin() + i + chunk);
+            i += chunk;
+            int rc;
+            if (chunk == 1) {
+                ServerRequest *r = part[0];
+                const stntt_request &q = r->req;
+                rc = gl ? stntt_ntt_goldilocks(W.ctx, (uint64_t *)r->d_buf, q.log2n, 1, q.direction, s)
+                        : stntt_ntt_u32(W.ctx, (uint32_t *)r->d_buf, q.log2n, 1, q.root, q.mod, q.direction, q.flags,
+                                        s);
+            } else {
+                void **tab = W.d_tab + ntab;
+                for (uint64_t j = 0; j < chunk; ++j) W.h_tab[ntab + j] = part[j]->d_buf;
+                HIP_TRY(hipMemcpyAsync(tab, W.h_tab + ntab, chunk * sizeof(void *), hipMemcpyHostToDevice, s));
+                ntab += (uint32_t)chunk;
+                const Plan *plan;
+                if ((rc = get_plan(W.ctx, std::get<0>(k), q0.log2n, std::get<4>(k), std::get<5>(k), q0.direction,
+                                   &plan)))
+                    return rc;
+                if (gl)
+                    rc = run_transform<FieldGL>(W.ctx, *plan, nullptr, chunk, std::get<3>(k), s, nullptr,
+                                                (const uint64_t *const *)tab, (uint64_t *const *)tab,
+                                                (uint64_t *)W.ctx->scratch);
+                else
+                    rc = run_transform<FieldU32>(W.ctx, *plan, nullptr, chunk, std::get<3>(k), s, nullptr,
+                                                 (const uint32_t *const *)tab, (uint32_t *const *)tab,
+                                                 (uint32_t *)W.ctx->scratch);
+                S->grouped_requests += chunk;
+            }
+            if (rc) return rc;
+            ServerGroup g;
+            for (ServerRequest *r : part) g.ids.push_back(r->id);
+            if ((rc = download(g, part))) return rc;
+        }
     }
     return STNTT_OK;
 }
@@ -1801,6 +1865,8 @@ static void server_release(stntt_server *srv) {
         if (W.d_tasks) (void)hipFree(W.d_tasks);
         if (W.d_counter) (void)hipFree(W.d_counter);
         if (W.d_scratch) (void)hipFree(W.d_scratch);
+        if (W.h_tab) (void)hipHostFree(W.h_tab);
+        if (W.d_tab) (void)hipFree(W.d_tab);
         if (W.ctx) stntt_ctx_destroy(W.ctx);
     }
     srv->workers.clear();
@@ -1833,6 +1899,8 @@ static int server_setup(stntt_server *S, int device, uint32_t workers) {
         HIP_TRY(hipMalloc(&W.d_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
         HIP_TRY(hipMalloc(&W.d_counter, sizeof(uint32_t)));
         HIP_TRY(hipMalloc(&W.d_scratch, (size_t)SERVER_BATCH_GRID * (8ull << SERVER_BATCH_MAX_LOG)));
+        HIP_TRY(hipHostMalloc((void **)&W.h_tab, SERVER_MAX_ROUND * sizeof(void *)));
+        HIP_TRY(hipMalloc((void **)&W.d_tab, SERVER_MAX_ROUND * sizeof(void *)));
     }
     return STNTT_OK;
 }
@@ -1956,6 +2024,7 @@ int stntt_server_stats_get(stntt_server *srv, stntt_server_stats *out) {
     out->held_bytes = srv->used + srv->cached;
     out->rounds = srv->rounds_launched;
     out->batched = srv->batched_requests;
+    out->grouped = srv->grouped_requests;
     if (!srv->latencies.empty()) {
         std::vector<double> v = srv->latencies;
         std::sort(v.begin(), v.end());
@@ -1979,6 +2048,7 @@ int stntt_server_reset_stats(stntt_server *srv) {
     srv->peak_used = srv->used + srv->cached;
     srv->rounds_launched = 0;
     srv->batched_requests = 0;
+    srv->grouped_requests = 0;
     srv->any = false;
     return STNTT_OK;
 }

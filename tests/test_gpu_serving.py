@@ -122,6 +122,34 @@ def test_size_classes_share_the_budget():
     srv.close()
 
 
+def test_same_shape_requests_run_grouped():
+    """Requests of one shape admitted in one round run as one grouped
+    multi-pass run over pointer tables, in chunks that the worker's 64 MiB
+    ping-pong holds (Goldilocks 2^20: four per chunk, so six split 4 + 2;
+    2^21: two, so three split 2 + 1); a shape with one request runs alone.
+    Every result must equal the oracle's.  A 2^22 request goes first so the
+    burst behind it is admitted together."""
+    srv = S.Server(0, hbm_budget_bytes=RESERVE + (2 << 30), workers=1)
+    rng = np.random.default_rng(41)
+    shapes = ([("goldilocks", 22, False)] + [("goldilocks", 18, False)] * 12 + [("goldilocks", 20, True)] * 6 +
+              [("goldilocks", 21, False)] * 3 + [("u32", 17, False)] * 8 + [("u32", 19, True)] * 5 +
+              [("goldilocks", 19, True)])
+    reqs = []
+    for field, L, inv in shapes:
+        data, root, mod = make_request(srv, rng, field, L, inv)
+        reqs.append((field, L, inv, data, data.array.copy(), root, mod))
+    ids = [srv.submit(field, L, data, inverse=inv, root=root, mod=mod) for field, L, inv, data, _, root, mod in reqs]
+    for i in ids:
+        srv.wait(i)
+    st = srv.stats()
+    assert st["completed"] == len(ids)
+    assert st["grouped"] >= 2, st
+    for k, (field, L, inv, data, x, root, mod) in enumerate(reqs):
+        assert np.array_equal(data.array, expected(field, L, inv, x, root, mod)), (k, field, L, inv)
+        data.free()
+    srv.close()
+
+
 def test_concurrent_submitters():
     srv = S.Server(0, hbm_budget_bytes=2 * RESERVE + (1 << 30), workers=2)
     results, errors = [], []
