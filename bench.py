@@ -23,7 +23,7 @@ sample of their transforms).
 
 Clock state: the GPU clock ramps up over the first ~50 ms of load and drops
 when the host leaves it idle, so before the W warm-up steps the headline
-workload runs for --settle-s seconds (default 0.1 s, untimed); the line
+workload runs for --settle-s seconds (default 0.2 s, untimed); the line
 says so in "clock".
 """
 from __future__ import annotations
@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched leg without the per-transform plain comparison")
     ap.add_argument("--serve-count", type=int, default=400)
-    ap.add_argument("--settle-s", type=float, default=0.1,
+    ap.add_argument("--settle-s", type=float, default=0.2,
                     help="seconds of untimed transforms before the warm-up steps (GPU clock ramp)")
     return ap.parse_args()
 

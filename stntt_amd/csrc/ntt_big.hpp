@@ -1,13 +1,15 @@
 // ntt_big.hpp -- the large-radix pass: a radix-R Stockham step (R = 2^9 ..
-// 2^12) over 128 KiB tiles, run by a persistent workgroup per CU that
-// issues the next tile's loads before computing the current one.
+// 2^12; the schedules use 2^10 .. 2^12) over tiles of up to 2^14 words, run
+// by persistent workgroups that walk the tiles in an XCD-aware order.
 //
 // The pass applies (stage algebra in ntt_pass.hpp, restating
 // src/ntt/ntt_cpu.cpp:66-79)
 //     y[(R s + k) J + j] = w^(s J k) * sum_m x[s J + j + m n/R] w_R^(m k)
-// to a tile of C consecutive columns t = s J + j and all R rows, with
-// R C w = 128 KiB (w the word size), i.e. 2^24 points take two passes of
-// radix 4096 instead of three of radix 256.  The R-point DFT of a column is
+// to a tile of C consecutive columns t = s J + j and all R rows, R C <=
+// 2^14 words (128 KiB of Goldilocks words, 64 KiB of 32-bit words), so 2^24
+// points take two passes of radix 4096 instead of three of radix 256, and
+// narrower tiles (C / 2, C / 4) give more workgroups for smaller n.  The
+// R-point DFT of a column is
 // three register sub-steps of radix 2^l1, 2^l2, 2^l3 (l1 = 4) with two LDS
 // exchanges; sub-step 1 loads its rows straight from HBM and sub-step 3
 // stores straight to HBM:
