@@ -119,6 +119,16 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     constexpr int RS = HALF_ROWS ? r1 / 2 : r1;  // image rows between the m-th inputs of a sub-step 2 group
     constexpr bool INNER_LDS = tile_inner_lds<FP, LOGR, PERM>();
     static_assert(!INNER_LDS || (G1 == 1 && R % NT == 0), "inner twiddle table layout");
+    // LDS image index of (row, image column lc): XOR swizzle by the row.  The
+    // column-halves image of 4-byte words (16 columns, 64-byte rows, two per
+    // 32 banks) swizzles by row / 2 instead: by the row itself, sub-step 2
+    // lanes g2 and g2 + 16 of a half-wave met on one bank (512 conflict
+    // cycles per tile); by row / 2 every half-wave access is conflict-free
+    // and the XOR operand is still a per-thread or compile-time constant.
+    auto img = [](uint32_t row, uint32_t lc) -> uint32_t {
+        if constexpr (HALF_COLS && sizeof(T) == 4) return row * CL + (lc ^ ((row >> 1) & (CL - 1)));
+        else return row * CL + (lc ^ (row & (CL - 1)));
+    };
 
     const uint32_t tid = threadIdx.x;
     const uint32_t logn = a.log_n;
@@ -300,7 +310,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 #pragma unroll
         for (int m = 0; m < r2; ++m) {
             const uint32_t row = g2l + m * RS;
-            v[m] = lds[row * CL + (lc ^ (row & (CL - 1)))];
+            v[m] = lds[img(row, lc)];
         }
         dft_regs<FP, LR2, DIR>(v, roots2, a.rt);
         [[maybe_unused]] Tw cur, step;
@@ -343,7 +353,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
 #pragma unroll
                 for (int k = 0; k < r1; ++k) {
                     const uint32_t row = r1 * xo_g + k, lc = xo_c & (CL - 1);
-                    lds[row * CL + (lc ^ (row & (CL - 1)))] = xo[k];
+                    lds[img(row, lc)] = xo[k];
                 }
             }
             __syncthreads();
