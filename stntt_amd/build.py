@@ -27,7 +27,7 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-FIELDS = {"gl": "FieldGL", "u32": "FieldU32", "m64": "FieldM64"}
+FIELDS = {"gl": "FieldGL", "u32": "FieldU32", "u32l": "FieldU32L", "m64": "FieldM64"}
 
 
 def units() -> list[tuple[str, str, list[str]]]:

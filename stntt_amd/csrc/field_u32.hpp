@@ -46,6 +46,55 @@ U32_HD uint32_t u32_mont(uint32_t x, uint32_t wm, uint32_t p, uint32_t pinv) {
     return u < r ? u : r;
 }
 
+// ---------------------------------------------------------------------------
+// Lazy reduction for p < 2^30 (Harvey, "Faster arithmetic for number-
+// theoretic transforms", J. Symbolic Comput. 60, 2014): words between
+// operations lie in [0, 2p) instead of [0, p).  With 4p < 2^32 no sum or
+// biased difference wraps, and a Montgomery product of x < 4p by wm < p (or
+// of two words below 2p) is below 4p^2 < 2^32 p, so its quotient
+// (t + m p) / 2^32 is already below 2p: the final subtraction is dropped.
+// A butterfly's (a - b) w is one biased subtraction and a three-instruction
+// product (5 VALU) instead of a reduced subtraction and a reduced product
+// (9).  Words are brought to [0, p) once, where a transform stores its
+// output.
+// ---------------------------------------------------------------------------
+
+// a + b for a, b < 2p: result in [0, 2p)
+U32_HD uint32_t u32l_add(uint32_t a, uint32_t b, uint32_t p2) {
+    uint32_t s = a + b;  // < 4p < 2^32
+    uint32_t t = s - p2;
+    return s < t ? s : t;
+}
+
+// a - b for a, b < 2p: result in [0, 2p)
+U32_HD uint32_t u32l_sub(uint32_t a, uint32_t b, uint32_t p2) {
+    uint32_t d = a + p2 - b;  // in (0, 4p)
+    uint32_t t = d - p2;
+    return d < t ? d : t;
+}
+
+// x wm 2^-32 mod p in [0, 2p) for x wm < 2^32 p
+U32_HD uint32_t u32l_mont(uint32_t x, uint32_t wm, uint32_t p, uint32_t pinv) {
+    const uint64_t t = (uint64_t)x * wm;
+    const uint32_t m = (uint32_t)t * pinv;
+    return (uint32_t)(((uint64_t)m * p + t) >> 32);
+}
+
+// (a - b) wm 2^-32 mod p in [0, 2p) for a, b < 2p and wm < p: the difference
+// is biased by 2p into (0, 4p) and multiplied unreduced
+U32_HD uint32_t u32l_sub_mont(uint32_t a, uint32_t b, uint32_t wm, uint32_t p, uint32_t pinv) {
+    return u32l_mont(a + 2 * p - b, wm, p, pinv);
+}
+
+// [0, 2p) -> [0, p)
+U32_HD uint32_t u32l_canon(uint32_t x, uint32_t p) {
+    uint32_t t = x - p;
+    return x < t ? x : t;
+}
+
+// the lazy form applies: 4p < 2^32
+U32_HD bool u32_lazy_modulus(uint32_t p) { return p < (1u << 30); }
+
 // -p^-1 mod 2^32 for odd p: Newton's iteration doubles the correct low bits
 // (x = p is correct to 3 bits)
 inline uint32_t u32_neg_inverse(uint32_t p) {

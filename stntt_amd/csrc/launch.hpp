@@ -9,6 +9,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
+
+#include <type_traits>
 
 #include "ntt_batch.hpp"
 #include "ntt_big.hpp"
@@ -26,9 +29,49 @@ template <class FP, int DIR>
 void launch_pass_dir(bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
                      uint32_t grid, hipStream_t s);
 
+// The lazy 32-bit field (FieldU32L) has FieldU32's argument layout: the
+// same words, tables and constants.
+inline PassArgs<FieldU32L> lazy_args(const PassArgs<FieldU32> &a) {
+    PassArgs<FieldU32L> b;
+    b.in = a.in;
+    b.out = a.out;
+    b.hi = a.hi;
+    b.lo = a.lo;
+    b.log_n = a.log_n;
+    b.log_j = a.log_j;
+    b.log_rmax = a.log_rmax;
+    b.batches = a.batches;
+    b.tiles_per_transform = a.tiles_per_transform;
+    b.flags = a.flags;
+    b.rt = a.rt;
+    b.mul_in = a.mul_in;
+    b.in_tab = a.in_tab;
+    b.out_tab = a.out_tab;
+    return b;
+}
+
+// 32-bit transforms with p < 2^30 run the lazy kernels.  STNTT_U32_LAZY=0 in
+// the environment keeps every modulus on the reduced ones (same words; for
+// A/B timing).
+inline bool u32_use_lazy(uint32_t p) {
+    static const bool enabled = [] {
+        const char *e = getenv("STNTT_U32_LAZY");
+        return !(e && e[0] == '0');
+    }();
+    return enabled && u32_lazy_modulus(p);
+}
+
 template <class FP>
 inline void launch_pass(int dir, bool single, bool first, bool perm, bool narrow, int logr, const PassArgs<FP> &a,
                         uint32_t grid, hipStream_t s) {
+    if constexpr (std::is_same_v<FP, FieldU32>) {
+        if (u32_use_lazy(a.rt.p)) {
+            const PassArgs<FieldU32L> b = lazy_args(a);
+            if (dir == 0) launch_pass_dir<FieldU32L, 0>(single, first, perm, narrow, logr, b, grid, s);
+            else launch_pass_dir<FieldU32L, 1>(single, first, perm, narrow, logr, b, grid, s);
+            return;
+        }
+    }
     if (dir == 0) launch_pass_dir<FP, 0>(single, first, perm, narrow, logr, a, grid, s);
     else launch_pass_dir<FP, 1>(single, first, perm, narrow, logr, a, grid, s);
 }
@@ -44,6 +87,14 @@ void launch_big_dir(bool first, int logr, int logc, const PassArgs<FP> &a, uint3
 template <class FP>
 inline void launch_big(int dir, bool first, int logr, int logc, const PassArgs<FP> &a, uint32_t ntiles, int num_cus,
                        hipStream_t s) {
+    if constexpr (std::is_same_v<FP, FieldU32>) {
+        if (u32_use_lazy(a.rt.p)) {
+            const PassArgs<FieldU32L> b = lazy_args(a);
+            if (dir == 0) launch_big_dir<FieldU32L, 0>(first, logr, logc, b, ntiles, num_cus, s);
+            else launch_big_dir<FieldU32L, 1>(first, logr, logc, b, ntiles, num_cus, s);
+            return;
+        }
+    }
     if (dir == 0) launch_big_dir<FP, 0>(first, logr, logc, a, ntiles, num_cus, s);
     else launch_big_dir<FP, 1>(first, logr, logc, a, ntiles, num_cus, s);
 }

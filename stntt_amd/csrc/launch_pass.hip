@@ -1,14 +1,14 @@
 // launch_pass.hip -- the pass kernels of one field and one direction.
 //
 // Compiled once per (field, direction), with STNTT_TU_FIELD (FieldGL,
-// FieldU32 or FieldM64) and STNTT_TU_DIR (0 forward, 1 inverse) defined on
+// FieldU32, FieldU32L or FieldM64) and STNTT_TU_DIR (0 forward, 1 inverse) defined on
 // the command line, so that the template instances of the library compile
 // in parallel translation units.
 #include "launch.hpp"
 #include "plan.hpp"
 
 #if !defined(STNTT_TU_FIELD) || !defined(STNTT_TU_DIR)
-#error "compile with -DSTNTT_TU_FIELD=<FieldGL|FieldU32|FieldM64> -DSTNTT_TU_DIR=<0|1>"
+#error "compile with -DSTNTT_TU_FIELD=<FieldGL|FieldU32|FieldU32L|FieldM64> -DSTNTT_TU_DIR=<0|1>"
 #endif
 
 namespace stntt {

@@ -197,6 +197,36 @@ struct FieldU32 {
     static __device__ __forceinline__ Tw tw_one(const Rt &rt) { return rt.c32; }
 };
 
+// The 32-bit field for moduli below 2^30 with lazy reduction (field_u32.hpp):
+// the same twiddle words, tables and constants as FieldU32, data words in
+// [0, 2p) inside a transform.  The first pass brings raw inputs to [0, p) as
+// FieldU32 does; intermediate passes store [0, 2p) words; the last pass
+// (PASS_LAST, finish) returns them to [0, p), so every output word is the
+// one FieldU32 computes.  The launchers pick it for p < 2^30.
+struct FieldU32L : FieldU32 {
+    static __device__ __forceinline__ T add(T a, T b, const Rt &rt) { return u32l_add(a, b, 2 * rt.p); }
+    static __device__ __forceinline__ T sub(T a, T b, const Rt &rt) { return u32l_sub(a, b, 2 * rt.p); }
+    static __device__ __forceinline__ T mul(T a, Tw w, const Rt &rt) { return u32l_mont(a, w, rt.p, rt.pinv); }
+    // u32_mul_any's Barrett quotient is exact to one for any product below
+    // 2^64, so [0, 2p) operands give a [0, p) result
+    static __device__ __forceinline__ T mulg(T a, T b, const Rt &rt) { return u32_mul_any(a, b, rt.p, rt.mu); }
+    template <int LR, int DIR, int J>
+    static __device__ __forceinline__ T sub_root_mul(T x, T y, const Roots &r, const Rt &rt) {
+        if constexpr (J == 0) return u32l_sub(x, y, 2 * rt.p);
+        return u32l_sub_mont(x, y, r.w[J], rt.p, rt.pinv);
+    }
+    static __device__ __forceinline__ T load_in(T x, const Rt &rt, uint32_t flags) {
+        if (flags & PASS_FIRST) return u32_reduce_input(x, rt.p, rt.bar, rt.c32, (flags & PASS_SIGNED) != 0);
+        return x;
+    }
+    static __device__ __forceinline__ T finish(T x, const Rt &rt, uint32_t flags) {
+        if (flags & PASS_SCALE) x = u32l_mont(x, rt.ninv, rt.p, rt.pinv);
+        return u32l_canon(x, rt.p);
+    }
+    // running products of twiddles: both factors below 2p, product below 2p
+    static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) { return u32l_mont(hi, lo, rt.p, rt.pinv); }
+};
+
 // Any odd prime below 2^64 (field_m64.hpp): data and twiddles in Montgomery
 // form inside a transform, converted on the first load and the last store.
 struct FieldM64 {

@@ -82,6 +82,18 @@ __global__ void k_u32(const uint32_t *x, const uint32_t *wm, const uint32_t *ca,
     out[2 * n + i] = u32_sub(ca[i], cb[i], p);
 }
 
+// 32-bit field, lazy form: operands la, lb in [0, 2p), wm < p
+__global__ void k_u32l(const uint32_t *la, const uint32_t *lb, const uint32_t *wm, uint32_t *out, uint32_t n,
+                       uint32_t p, uint32_t pinv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = u32l_add(la[i], lb[i], 2 * p);
+    out[n + i] = u32l_sub(la[i], lb[i], 2 * p);
+    out[2 * n + i] = u32l_sub_mont(la[i], lb[i], wm[i], p, pinv);
+    out[3 * n + i] = u32l_mont(la[i], lb[i], p, pinv);
+    out[4 * n + i] = u32l_canon(la[i], p);
+}
+
 static int failures = 0;
 static long cases = 0;
 #define CHECK(cond, ...)                        \
@@ -261,6 +273,40 @@ int main() {
         HIP_CHECK(hipFree(dw));
         HIP_CHECK(hipFree(dxa));
         HIP_CHECK(hipFree(dxb));
+        HIP_CHECK(hipFree(dout));
+    }
+    // ---- 32-bit field, lazy form (p < 2^30) ----
+    for (uint32_t p : {3u, 17u, 7340033u, 998244353u, 1004535809u, 1073741789u}) {
+        const uint32_t pinv = u32_neg_inverse(p), p2 = 2 * p;
+        const uint32_t edge[] = {0, 1, p - 1, p, p + 1, p2 - 2, p2 - 1};
+        std::vector<uint32_t> la(n), lb(n), wm(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            la[i] = i < 49 ? edge[i % 7] : (uint32_t)(a[i] % p2);
+            lb[i] = i < 49 ? edge[i / 7] : (uint32_t)(b[i] % p2);
+            wm[i] = (uint32_t)((a[i] ^ b[i]) % p);
+        }
+        uint32_t *dla = dev_copy(la), *dlb = dev_copy(lb), *dw = dev_copy(wm), *dout = nullptr;
+        HIP_CHECK(hipMalloc(&dout, 5ull * n * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_u32l, grid, block, 0, 0, dla, dlb, dw, dout, n, p, pinv);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        const auto out = host_copy(dout, 5ull * n);
+        const uint64_t r32 = (1ull << 32) % p;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t x = la[i], y = lb[i];
+            CHECK(out[i] < p2 && out[i] % p == (x + y) % p, "u32l_add p %u", p);
+            CHECK(out[n + i] < p2 && out[n + i] % p == (x + p2 - y) % p, "u32l_sub p %u", p);
+            // r 2^32 == value (mod p), r < 2p
+            CHECK(out[2 * n + i] < p2 && (uint64_t)((u128)out[2 * n + i] * r32 % p) ==
+                                             (uint64_t)((u128)((x + p2 - y) % p) * wm[i] % p),
+                  "u32l_sub_mont p %u", p);
+            CHECK(out[3 * n + i] < p2 && (uint64_t)((u128)out[3 * n + i] * r32 % p) == (uint64_t)((u128)x * y % p),
+                  "u32l_mont p %u", p);
+            CHECK(out[4 * n + i] == x % p, "u32l_canon p %u", p);
+        }
+        HIP_CHECK(hipFree(dla));
+        HIP_CHECK(hipFree(dlb));
+        HIP_CHECK(hipFree(dw));
         HIP_CHECK(hipFree(dout));
     }
     std::printf("device field test: %ld cases, %d failures\n", cases, failures);

@@ -109,6 +109,42 @@ int main() {
             }
         }
     }
+    // 32-bit field, lazy form (p < 2^30): operands anywhere in [0, 2p),
+    // results in [0, 2p) and congruent to the exact value
+    for (uint32_t p : {3u, 11u, 65537u, 7340033u, 998244353u, 1004535809u, 1053818881u, 1073741789u}) {
+        CHECK(u32_lazy_modulus(p), "lazy modulus %u", p);
+        const uint32_t pinv = u32_neg_inverse(p), p2 = 2 * p;
+        std::vector<uint32_t> ls = {0, 1, p - 1, p, p + 1, p2 - 2, p2 - 1, p / 2, p + p / 2};
+        for (int i = 0; i < 160; ++i) ls.push_back((uint32_t)(rng() % p2));
+        std::vector<uint32_t> ws = {0, 1, p - 1, p / 2};
+        for (int i = 0; i < 12; ++i) ws.push_back((uint32_t)(rng() % p));
+        const uint64_t r32 = (1ull << 32) % p;
+        auto mont_ok = [&](uint32_t r, uint64_t prod) {  // r 2^32 == prod (mod p)
+            return r < p2 && (uint64_t)(((u128)r * r32) % p) == (uint64_t)(prod % p);
+        };
+        for (uint32_t a : ls) {
+            CHECK(u32l_canon(a, p) == a % p, "u32l canon %u mod %u", a, p);
+            for (uint32_t b : ls) {
+                const uint32_t s = u32l_add(a, b, p2), d = u32l_sub(a, b, p2);
+                CHECK(s < p2 && s % p == (uint32_t)(((uint64_t)a + b) % p), "u32l add %u %u mod %u", a, b, p);
+                CHECK(d < p2 && d % p == (uint32_t)(((uint64_t)a + p2 - b) % p), "u32l sub %u %u mod %u", a, b, p);
+                // two lazy words (running twiddle products), and u32_mul_any on lazy words
+                CHECK(mont_ok(u32l_mont(a, b, p, pinv), (u128)a * b % p), "u32l mont %u %u mod %u", a, b, p);
+                CHECK(u32_mul_any(a, b, p, ~0ull / p) == (uint32_t)((uint64_t)a * b % p), "u32 mul_any lazy %u %u", a, b);
+                for (uint32_t w : ws) {
+                    const uint64_t diff = ((uint64_t)a + p2 - b) % p;
+                    CHECK(mont_ok(u32l_sub_mont(a, b, w, p, pinv), (u128)diff * w), "u32l sub_mont %u %u %u mod %u",
+                          a, b, w, p);
+                }
+            }
+            // a biased difference (below 4p) times a table word
+            for (uint32_t w : ws) {
+                const uint32_t x = a + p2 - 1;
+                CHECK(mont_ok(u32l_mont(x, w, p, pinv), (u128)x * w), "u32l mont 4p %u %u mod %u", x, w, p);
+            }
+        }
+    }
+    CHECK(!u32_lazy_modulus(1085276161u) && !u32_lazy_modulus(2147483647u), "lazy modulus bound");
     // 64-bit Montgomery field: primes below 2^32, between 2^32 and 2^63,
     // above 2^63, Goldilocks and the largest 64-bit prime
     for (uint64_t p : std::vector<uint64_t>{65537ull, 4294967291ull, 1099511627791ull, 0x3FFFFFFFFFFFFFFDull,

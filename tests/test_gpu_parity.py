@@ -228,11 +228,16 @@ def test_u32_large(ctx, L):
 
 # The 32-bit field's twiddles are Montgomery words (REDC with -p^-1 mod 2^32):
 # moduli at both ends of its range, through every kernel family (single-pass,
-# tile passes, the batch queue).  root = None: findParams from that modulus up
-# (each listed modulus is the prime it finds).
+# tile passes, large-radix passes, the batch queue).  Below 2^30 the pass
+# kernels keep words in [0, 2p) (lazy reduction), from 2^30 on in [0, p): the
+# moduli just below and just above 2^30 cover both sides of that switch.
+# root = None: findParams from that modulus up (each listed modulus is the
+# prime it finds).
 EXTREME_MODULI = [(1, 3, 2), (2, 5, 2), (4, 17, 3), (1, 2**31 - 1, 2**31 - 2),
                   (10, 2145389569, None), (14, 2013265921, None), (16, 2130706433, None),
-                  (20, 1888485377, None)]
+                  (20, 1888485377, None),
+                  (12, 1073479681, None), (18, 1073479681, None), (20, 1053818881, None),
+                  (21, 1004535809, None), (20, 1085276161, None), (12, 1073750017, None)]
 
 
 @pytest.mark.parametrize("L,mod,root", EXTREME_MODULI)
