@@ -32,6 +32,7 @@ void launch_pass_dir(bool single, bool first, bool perm, bool narrow, int logr, 
 // The lazy 32-bit field (FieldU32L) has FieldU32's argument layout: the
 // same words, tables and constants.
 inline PassArgs<FieldU32L> lazy_args(const PassArgs<FieldU32> &a) {
+    static_assert(sizeof(PassArgs<FieldU32L>) == sizeof(PassArgs<FieldU32>), "copy every PassArgs field below");
     PassArgs<FieldU32L> b;
     b.in = a.in;
     b.out = a.out;

@@ -218,6 +218,9 @@ __global__ void __launch_bounds__((big_threads<FP, LOGR, LOGC>()))
                     const uint32_t e0 = e * g, es = e * (uint32_t)(R / r3);  // < n
                     cur = FP::twiddle(at_off(a.hi, e0 >> log_b), at_off(a.lo, e0 & bmask), a.rt);
                     step = FP::twiddle(at_off(a.hi, es >> log_b), at_off(a.lo, es & bmask), a.rt);
+                    if constexpr (FP::SCALE_IN) {
+                        if (J1 && (a.flags & PASS_SCALE_IN)) cur = FP::twiddle(cur, FP::tw_ninv(a.rt), a.rt);
+                    }
                 }
                 dft_regs<FP, L3, DIR>(u[gi], roots3, a.rt);
                 const uint32_t lane_out = J1 ? (c * R + g) : ((g << a.log_j) + c);

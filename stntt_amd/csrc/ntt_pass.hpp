@@ -69,7 +69,12 @@ enum : uint32_t {
     PASS_FIRST = 1u,   // reduce raw inputs (32-bit field)
     PASS_LAST = 2u,    // canonicalise outputs, apply n^-1 for the inverse
     PASS_SIGNED = 4u,  // 32-bit inputs are signed ints (reference int storage)
-    PASS_SCALE = 8u,   // multiply by n^-1 (inverse)
+    PASS_SCALE = 8u,   // multiply by n^-1 (inverse), on the last pass's store
+    // multiply by n^-1 (inverse) through the first pass's pass twiddle: a J =
+    // 1 pass multiplies every output word by its running-product twiddle, so
+    // starting that product at n^-1 scales the whole transform for one
+    // product per thread instead of one per word (fields with SCALE_IN)
+    PASS_SCALE_IN = 64u,
     PASS_BITREV_IN = 16u,   // input in bit-reversed order (single-pass kernel)
     PASS_BITREV_OUT = 32u,  // output in bit-reversed order (single-pass kernel)
 };
@@ -145,6 +150,9 @@ struct FieldGL {
     // twiddle hi * lo from the two factor tables
     static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &) { return gl_mul(hi, lo); }
     static __device__ __forceinline__ Tw tw_one(const Rt &) { return 1; }
+    // n^-1 as a twiddle word (PASS_SCALE_IN)
+    static constexpr bool SCALE_IN = true;
+    static __device__ __forceinline__ Tw tw_ninv(const Rt &rt) { return rt.ninv; }
 };
 
 struct FieldU32 {
@@ -195,6 +203,9 @@ struct FieldU32 {
     // twiddle hi * lo: the Montgomery product of two Montgomery words
     static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) { return u32_mont(hi, lo, rt.p, rt.pinv); }
     static __device__ __forceinline__ Tw tw_one(const Rt &rt) { return rt.c32; }
+    // n^-1 as a twiddle word (PASS_SCALE_IN): rt.ninv is a Montgomery word
+    static constexpr bool SCALE_IN = true;
+    static __device__ __forceinline__ Tw tw_ninv(const Rt &rt) { return rt.ninv; }
 };
 
 // The 32-bit field for moduli below 2^30 with lazy reduction (field_u32.hpp):
@@ -266,6 +277,10 @@ struct FieldM64 {
     }
     static __device__ __forceinline__ Tw twiddle(Tw hi, Tw lo, const Rt &rt) { return m64_mul(hi, lo, rt.p, rt.pinv); }
     static __device__ __forceinline__ Tw tw_one(const Rt &rt) { return rt.r1; }
+    // the last pass's conversion out of Montgomery form is a product anyway
+    // and takes n^-1 for free, so the scale stays there
+    static constexpr bool SCALE_IN = false;
+    static __device__ __forceinline__ Tw tw_ninv(const Rt &rt) { return rt.r1; }
 };
 
 // ---------------------------------------------------------------------------

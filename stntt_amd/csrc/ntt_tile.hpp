@@ -317,6 +317,9 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         if constexpr (J1) {
             cur = FP::twiddle(th[0], tl[0], a.rt);
             step = FP::twiddle(th[1], tl[1], a.rt);
+            if constexpr (FP::SCALE_IN) {
+                if (a.flags & PASS_SCALE_IN) cur = FP::twiddle(cur, FP::tw_ninv(a.rt), a.rt);  // uniform
+            }
         }
         // stores: a uniform base per k2 plus one 32-bit lane offset
         const uint32_t lane_out = J1 ? (c * R + g2) : ((g2 << a.log_j) + c);

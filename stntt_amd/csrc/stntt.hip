@@ -556,8 +556,14 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         a.in = src;
         a.out = dst;
         a.log_j = log_j;
-        a.flags = (first ? (PASS_FIRST | (flags & STNTT_SIGNED_INPUT ? PASS_SIGNED : 0)) : 0) |
-                  (last ? (PASS_LAST | (inv ? PASS_SCALE : 0)) : 0);
+        // the inverse's n^-1: through the first pass's twiddles where the
+        // field allows it (every J = 1 output takes a twiddle product),
+        // otherwise on the last pass's store
+        const bool scale_in = inv && !sch.single && FP::SCALE_IN;
+        a.flags = (first ? (PASS_FIRST | (flags & STNTT_SIGNED_INPUT ? PASS_SIGNED : 0) |
+                            (scale_in ? PASS_SCALE_IN : 0))
+                         : 0) |
+                  (last ? (PASS_LAST | (inv && !scale_in ? PASS_SCALE : 0)) : 0);
         a.mul_in = first ? mul_in : nullptr;
         a.in_tab = first ? in_tab : nullptr;
         a.out_tab = last ? out_tab : nullptr;
