@@ -14,6 +14,11 @@ buffer once: 2 n w = 256 MiB) / its mean duration, against the 8 TB/s HBM3E
 peak.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE
 (recorded in profiles/; null here when not supplied).
 
+hbm_copy: the library's plain copy kernel moving the same 2^24 x 8 B buffer
+over four buffers in turn (512 MiB, beyond the 256 MB MALL, as the passes'
+ping-pong is): the HBM rate this footprint reaches in practice, with the
+dominant pass and the whole transform as fractions of it.
+
 cpu_baseline: the oracle's restatement of the reference's CPU Stockham
 (src/ntt/ntt_cpu.cpp:46-90 with exact Goldilocks arithmetic), single
 thread, on a bounded sample of the same workload; the host's core count is
@@ -136,6 +141,31 @@ def cpu_all_cores(specs, budget_s: float, label: str) -> dict:
     return {"transforms_s": round(done / dt, 1), "gelem_s": round(elems / dt / 1e9, 4), "cores": cores,
             "kind": "port", "sample": f"first {done} transforms of the {label} ({elems} elements) in "
                                       f"{dt:.1f} s on {cores} threads (oracle/ntt_oracle.c)"}
+
+
+def copy_rate(ctx, nbytes: int, reps: int = 200) -> float:
+    """HBM rate of the library's plain copy kernel (one 8-byte word per
+    lane), nbytes -> nbytes over four buffers walked in turn: 4 nbytes of
+    footprint, more than the 256 MB MALL at nbytes = 128 MiB, so (like the
+    ping-pong of a transform's passes) every pass streams from HBM.
+    Returns TB/s of read + write bytes."""
+    bufs = [ctx.alloc(nbytes) for _ in range(4)]
+    try:
+        t_end = time.perf_counter() + 0.05
+        i = 0
+        while time.perf_counter() < t_end:
+            ctx.copy(bufs[(i + 1) & 3], bufs[i & 3], nbytes)
+            i += 1
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(reps):
+            ctx.copy(bufs[(i + 1) & 3], bufs[i & 3], nbytes)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        for b in bufs:
+            b.free()
+    return 2 * nbytes / dt / 1e12
 
 
 def bench_n20(S, ctx, steps: int) -> dict:
@@ -558,6 +588,12 @@ def main() -> int:
             "instr_per_launch": valu_instr},
         "pass_ms": [round(v, 5) for v in mean_pass],
     }
+    # what a plain copy of the same buffer reaches when, like the passes,
+    # it cannot stay in the MALL: the HBM rate this access footprint allows
+    copy_tbs = copy_rate(ctx, n * 8)
+    out["hbm_copy"] = {"tbs": round(copy_tbs, 3), "bytes": 2 * n * 8, "footprint_bytes": 4 * n * 8,
+                       "dominant_pass_frac_of_copy": round(achieved / 1e3 / copy_tbs, 4),
+                       "transform_frac_of_copy": round(npass * alg_bytes / (transform_ms * 1e-3) / 1e12 / copy_tbs, 4)}
     legs = set() if args.no_extra else set(args.legs.split(","))
     if "n20" in legs:
         log("[bench] n=2^20 forward+inverse, both fields")
