@@ -517,8 +517,17 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     const bool rev_in = flags & STNTT_BITREV_IN, rev_out = flags & STNTT_BITREV_OUT;
     if (mul_in && !sch.single && !rev_in)
         return fail(STNTT_EINVAL, "a fused pointwise operand needs bit-reversed input on multi-pass sizes");
-    // intermediates alternate between two scratch halves so that the last
-    // pass always lands in the caller's buffer (no copy-back)
+    // Where the passes write.  Contiguous data: the passes alternate between
+    // the caller's buffer and one scratch buffer, and an odd count ends with
+    // the last pass in place: it reads x[j + m J] and writes y[k J + j]
+    // (J = n / R), the same positions, each tile its own columns, every
+    // load of a tile before any of its stores.  Two buffers of n w bytes
+    // instead of three, so a 2^24 Goldilocks transform's 256 MiB stays in
+    // the 256 MB MALL between passes.  Gathered tables and bit-reversed
+    // output (whose last pass writes other positions than it reads) keep
+    // two scratch halves that the passes alternate between, the last pass
+    // landing in the caller's buffers (no copy-back either way).
+    const bool two_buf = !in_tab && !out_tab && !rev_out;
     T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
@@ -552,7 +561,9 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     for (int p = 0; p < sch.npasses; ++p) {
         const int logr = sch.logr[p];
         const bool first = p == 0, last = p == sch.npasses - 1;
-        T *dst = last ? d_data : scratch[p & 1];
+        T *dst;
+        if (two_buf) dst = (last || (p & 1)) ? d_data : scratch[0];
+        else dst = last ? d_data : scratch[p & 1];
         a.in = src;
         a.out = dst;
         a.log_j = log_j;
