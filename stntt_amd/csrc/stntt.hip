@@ -523,11 +523,12 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     // (J = n / R), the same positions, each tile its own columns, every
     // load of a tile before any of its stores.  Two buffers of n w bytes
     // instead of three, so a 2^24 Goldilocks transform's 256 MiB stays in
-    // the 256 MB MALL between passes.  Gathered tables and bit-reversed
-    // output (whose last pass writes other positions than it reads) keep
-    // two scratch halves that the passes alternate between, the last pass
-    // landing in the caller's buffers (no copy-back either way).
-    const bool two_buf = !in_tab && !out_tab && !rev_out;
+    // the 256 MB MALL between passes.  Gathered tables, and bit-reversed
+    // output over an odd pass count (that last pass writes other positions
+    // than it reads), keep two scratch halves that the passes alternate
+    // between, the last pass landing in the caller's buffers (no copy-back
+    // either way).
+    const bool two_buf = !in_tab && !out_tab && !(rev_out && (sch.npasses & 1));
     T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
