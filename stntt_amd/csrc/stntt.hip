@@ -517,18 +517,18 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     const bool rev_in = flags & STNTT_BITREV_IN, rev_out = flags & STNTT_BITREV_OUT;
     if (mul_in && !sch.single && !rev_in)
         return fail(STNTT_EINVAL, "a fused pointwise operand needs bit-reversed input on multi-pass sizes");
-    // Where the passes write.  Contiguous data: the passes alternate between
-    // the caller's buffer and one scratch buffer, and an odd count ends with
-    // the last pass in place: it reads x[j + m J] and writes y[k J + j]
-    // (J = n / R), the same positions, each tile its own columns, every
-    // load of a tile before any of its stores.  Two buffers of n w bytes
-    // instead of three, so a 2^24 Goldilocks transform's 256 MiB stays in
-    // the 256 MB MALL between passes.  Gathered tables, and bit-reversed
-    // output over an odd pass count (that last pass writes other positions
-    // than it reads), keep two scratch halves that the passes alternate
-    // between, the last pass landing in the caller's buffers (no copy-back
-    // either way).
-    const bool two_buf = !in_tab && !out_tab && !(rev_out && (sch.npasses & 1));
+    // Where the passes write.  The passes alternate between the caller's
+    // buffers (d_data, or the buffers of out_tab) and one scratch buffer,
+    // and an odd count ends with the last pass in place: it reads
+    // x[j + m J] and writes y[k J + j] (J = n / R), the same positions, each
+    // tile its own columns, every load of a tile before any of its stores.
+    // Two buffers of n w bytes per transform instead of three, so a 2^24
+    // Goldilocks transform's 256 MiB stays in the 256 MB MALL between
+    // passes.  Bit-reversed output over an odd pass count (that last pass
+    // writes other positions than it reads) keeps two scratch halves that
+    // the passes alternate between, the last pass landing in the caller's
+    // buffers.  No copy-back either way.
+    const bool two_buf = !(rev_out && (sch.npasses & 1));
     T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
@@ -552,7 +552,8 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     else if constexpr (std::is_same_v<FP, FieldU32>) a.rt = plan.rt_u32;
     else a.rt = plan.rt_m64;
 
-    T *src = d_data;
+    bool src_caller = true;  // the pass reads the caller's buffers (else src)
+    T *src = nullptr;
     uint32_t log_j = 0;
     const bool prof = ctx->profile && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
     if (prof) {
@@ -562,11 +563,14 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     for (int p = 0; p < sch.npasses; ++p) {
         const int logr = sch.logr[p];
         const bool first = p == 0, last = p == sch.npasses - 1;
-        T *dst;
-        if (two_buf) dst = (last || (p & 1)) ? d_data : scratch[0];
-        else dst = last ? d_data : scratch[p & 1];
-        a.in = src;
-        a.out = dst;
+        const bool dst_caller = two_buf ? (last || (p & 1)) : last;
+        T *dst = dst_caller ? nullptr : scratch[two_buf ? 0 : (p & 1)];
+        // the caller's input on the first pass (in_tab or d_data), its
+        // output buffers on later ones (out_tab or d_data)
+        a.in = src_caller ? d_data : src;
+        a.in_tab = src_caller ? (first ? in_tab : out_tab) : nullptr;
+        a.out = dst_caller ? d_data : dst;
+        a.out_tab = dst_caller ? out_tab : nullptr;
         a.log_j = log_j;
         // the inverse's n^-1: through the first pass's twiddles where the
         // field allows it (every J = 1 output takes a twiddle product),
@@ -577,8 +581,6 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
                          : 0) |
                   (last ? (PASS_LAST | (inv && !scale_in ? PASS_SCALE : 0)) : 0);
         a.mul_in = first ? mul_in : nullptr;
-        a.in_tab = first ? in_tab : nullptr;
-        a.out_tab = last ? out_tab : nullptr;
         bool perm = false;
         if (sch.single) {
             a.flags |= (rev_in ? PASS_BITREV_IN : 0) | (rev_out ? PASS_BITREV_OUT : 0);
@@ -606,6 +608,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             launch_pass<FP>(plan.key.dir, sch.single, first, perm, narrow, logr, a, grid, s);
         HIP_TRY(hipGetLastError());
         if (prof) HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
+        src_caller = dst_caller;
         src = dst;
         log_j += logr;
     }
