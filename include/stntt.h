@@ -117,8 +117,10 @@ int stntt_ntt_goldilocks_ex(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, ui
    array of the library (256 KiB of tables + 64 MiB of ping-pong per device,
    held until the process ends); each call regenerates its tables there, so
    a call -- or a replay of a captured call -- stands alone.  Limit: log2n
-   <= 28 and a schedule whose ping-pong fits 64 MiB (one transform up to
-   2^22, or 16 of 2^18; larger is STNTT_EINVAL).  Every call on a device
+   <= 28 and a schedule whose scratch fits 64 MiB (one transform up to
+   2^23, or 32 of 2^18, in natural order or with bit-reversed input; with
+   bit-reversed output over three passes half that; larger is
+   STNTT_EINVAL).  Every call on a device
    shares that device's array and keeps the last call's contents: calls
    that overlap on one device (several streams or threads at once) race. */
 int stntt_ntt_goldilocks_noalloc(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2n, uint64_t batches, int direction,

@@ -464,12 +464,21 @@ static Schedule transform_schedule(const stntt_ctx *ctx, int field, uint32_t L, 
     return sch;
 }
 
-// ping-pong bytes a schedule needs besides the data (no copy-back: the last
-// pass writes the caller's buffer)
+// Whether a multi-pass schedule runs on two buffers (the caller's and one
+// scratch; an odd pass count ends with the last pass in place): all but
+// bit-reversed output over an odd pass count, whose last pass writes other
+// positions than it reads.
+static bool two_buffer_schedule(const Schedule &sch, uint32_t flags) {
+    return !((flags & STNTT_BITREV_OUT) && (sch.npasses & 1));
+}
+
+// scratch bytes a schedule needs besides the data (no copy-back: the last
+// pass writes the caller's buffer): one buffer of the data's size, two for
+// a three-buffer schedule
 template <class FP>
-static uint64_t schedule_scratch_bytes(const Schedule &sch, uint64_t total_words) {
+static uint64_t schedule_scratch_bytes(const Schedule &sch, uint64_t total_words, uint32_t flags) {
     if (sch.npasses < 2) return 0;
-    return (sch.npasses > 2 ? 2 : 1) * total_words * sizeof(typename FP::T);
+    return (sch.npasses > 2 && !two_buffer_schedule(sch, flags) ? 2 : 1) * total_words * sizeof(typename FP::T);
 }
 
 // Runs the pass schedule of `plan` over d_data[batches][n] in place.
@@ -528,7 +537,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     // writes other positions than it reads) keeps two scratch halves that
     // the passes alternate between, the last pass landing in the caller's
     // buffers.  No copy-back either way.
-    const bool two_buf = !(rev_out && (sch.npasses & 1));
+    const bool two_buf = two_buffer_schedule(sch, flags);
     T *scratch[2] = {nullptr, nullptr};
     const uint64_t total = batches << L;
     if (sch.npasses > 1) {
@@ -536,11 +545,11 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
             scratch[0] = ext_scratch;
         } else {
             void *p = nullptr;
-            int rc = acquire_scratch(ctx, schedule_scratch_bytes<FP>(sch, total), s, &p);
+            int rc = acquire_scratch(ctx, schedule_scratch_bytes<FP>(sch, total, flags), s, &p);
             if (rc) return rc;
             scratch[0] = (T *)p;
         }
-        scratch[1] = sch.npasses > 2 ? scratch[0] + total : nullptr;
+        scratch[1] = two_buf ? nullptr : scratch[0] + total;
     }
     PassArgs<FP> a{};
     a.hi = (const typename FP::Tw *)plan.hi();
@@ -964,7 +973,7 @@ template <class FP>
 static uint64_t server_chunk(const stntt_ctx *ctx, int field, uint32_t L, uint64_t want) {
     uint64_t c = std::min<uint64_t>(want, SERVER_MAX_ROUND);
     while (c > 1) {
-        const uint64_t need = schedule_scratch_bytes<FP>(transform_schedule<FP>(ctx, field, L, c), c << L);
+        const uint64_t need = schedule_scratch_bytes<FP>(transform_schedule<FP>(ctx, field, L, c), c << L, 0);
         if (need <= ctx->scratch_bytes) break;
         c = std::max<uint64_t>(1, std::min<uint64_t>(c - 1, ctx->scratch_bytes / (need / c)));
     }
@@ -1482,8 +1491,9 @@ int stntt_ntt_goldilocks_noalloc(stntt_ctx *ctx, uint64_t *d_data, uint32_t log2
     if (batches >= (1ull << 31) || (batches << log2n) >> log2n != batches)
         return fail(STNTT_EINVAL, "%llu transforms of 2^%u words is more than one call handles",
                     (unsigned long long)batches, log2n);
-    const Schedule sch = transform_schedule<FieldGL>(ctx, STNTT_FIELD_GOLDILOCKS, log2n, batches);
-    const uint64_t need = schedule_scratch_bytes<FieldGL>(sch, batches << log2n);
+    // the schedule run_transform will run (order flags rule out large-radix passes)
+    const Schedule sch = transform_schedule<FieldGL>(ctx, STNTT_FIELD_GOLDILOCKS, log2n, batches, flags);
+    const uint64_t need = schedule_scratch_bytes<FieldGL>(sch, batches << log2n, flags);
     if (need > NOALLOC_DATA_BYTES)
         return fail(STNTT_EINVAL,
                     "%llu transforms of 2^%u words need %llu bytes of scratch; the allocation-free path holds %llu",

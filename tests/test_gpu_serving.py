@@ -125,14 +125,15 @@ def test_size_classes_share_the_budget():
 def test_same_shape_requests_run_grouped():
     """Requests of one shape admitted in one round run as one grouped
     multi-pass run over pointer tables, in chunks that the worker's 64 MiB
-    ping-pong holds (Goldilocks 2^20: four per chunk, so six split 4 + 2;
-    2^21: two, so three split 2 + 1); a shape with one request runs alone.
+    scratch holds (two-buffer schedule: Goldilocks 2^20 eight per chunk, so
+    ten split 8 + 2; 2^21 four, so five split 4 + 1); a shape with one
+    request runs alone.
     Every result must equal the oracle's.  A 2^22 request goes first so the
     burst behind it is admitted together."""
     srv = S.Server(0, hbm_budget_bytes=RESERVE + (2 << 30), workers=1)
     rng = np.random.default_rng(41)
-    shapes = ([("goldilocks", 22, False)] + [("goldilocks", 18, False)] * 12 + [("goldilocks", 20, True)] * 6 +
-              [("goldilocks", 21, False)] * 3 + [("u32", 17, False)] * 8 + [("u32", 19, True)] * 5 +
+    shapes = ([("goldilocks", 22, False)] + [("goldilocks", 18, False)] * 12 + [("goldilocks", 20, True)] * 10 +
+              [("goldilocks", 21, False)] * 5 + [("u32", 17, False)] * 8 + [("u32", 19, True)] * 5 +
               [("goldilocks", 19, True)])
     reqs = []
     for field, L, inv in shapes:

@@ -38,7 +38,8 @@ def ctx():
     _lib.lib().stntt_ctx_destroy(h)
 
 
-@pytest.mark.parametrize("sizes", [[(22, 1), (12, 3)], [(18, 16), (16, 2)], [(20, 2), (9, 5)], [(13, 1), (5, 2)]])
+@pytest.mark.parametrize("sizes", [[(23, 1), (18, 16)], [(22, 1), (12, 3)], [(18, 16), (16, 2)], [(20, 2), (9, 5)],
+                                   [(13, 1), (5, 2)]])
 def test_noalloc_matches_allocating_entry_point(ctx, sizes):
     rng = np.random.default_rng(sum(L for L, _ in sizes))
     for L, batches in sizes:  # two calls in a row, the second smaller
@@ -64,7 +65,12 @@ def test_noalloc_orders(ctx):
 
 def test_noalloc_rejects_sizes_past_its_scratch(ctx):
     L_ = _lib.lib()
-    # 2^24 words need 128 MiB or more of ping-pong: rejected before any device work
+    # 2^24 words need 128 MiB or more of scratch: rejected before any device work
     assert L_.stntt_ntt_goldilocks_noalloc(ctx, _lib._c.c_void_p(1 << 20), 24, 1, 0, 0, None) == _lib.STNTT_EINVAL
     assert b"allocation-free" in L_.stntt_last_error()
     assert L_.stntt_ntt_goldilocks_noalloc(ctx, _lib._c.c_void_p(1 << 20), 29, 1, 0, 0, None) == _lib.STNTT_EINVAL
+    # 2^23 fits in natural order (three passes over the data and one 64 MiB
+    # scratch) but not with bit-reversed output, whose odd pass count needs
+    # two scratch buffers
+    assert L_.stntt_ntt_goldilocks_noalloc(ctx, _lib._c.c_void_p(1 << 20), 23, 1, 0, _lib.BITREV_OUT,
+                                           None) == _lib.STNTT_EINVAL
