@@ -158,6 +158,21 @@ struct stntt_batch {
     uint64_t *d_scratch = nullptr;
     uint64_t scratch_stride = 0;  // 64-bit words per workgroup
     uint32_t grid = 0;
+    // The groups and the queue launch are independent of each other: they
+    // run on up to MAX_LANES streams at once, lane 0 being the run's own
+    // stream and the others the batch's, forked from it and joined back into
+    // it by events (captured as parallel branches of the graph).  Work is
+    // dealt to lanes largest first, each to the least loaded lane; each lane
+    // has its own ping-pong scratch, at lane_off[lane] in d_group_scratch.
+    static constexpr int MAX_LANES = 4;
+    int lanes = 1;
+    hipStream_t lane_stream[MAX_LANES] = {};  // [0] unused: the run's stream
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[MAX_LANES] = {};
+    int queue_lane = 0;                  // lane of the queue launch
+    std::vector<int> group_lane;         // lane of each group
+    std::vector<size_t> launch_order;    // groups, largest first
+    size_t lane_off[MAX_LANES] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // runs share the queue counter and the group scratch: a run on another
@@ -564,7 +579,9 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     bool src_caller = true;  // the pass reads the caller's buffers (else src)
     T *src = nullptr;
     uint32_t log_j = 0;
-    const bool prof = ctx->profile && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
+    // per-pass events for a single caller's transform only (grouped runs of
+    // a batch go out on several streams at once)
+    const bool prof = ctx->profile && !in_tab && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
     if (prof) {
         ctx->prof_count = 0;
         HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
@@ -829,7 +846,6 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
     if (!tasks.empty())
         HIP_TRY(hipMemcpy(b->d_tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&b->d_counter, sizeof(uint32_t)));
-    size_t scratch = 0;
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         BatchGroup &g = b->groups[gi];
         g.count = (uint32_t)gsrc[gi].size();
@@ -838,37 +854,109 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
         HIP_TRY(hipMalloc((void **)&g.d_dst, tab));
         HIP_TRY(hipMemcpy(g.d_src, gsrc[gi].data(), tab, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(g.d_dst, gdst[gi].data(), tab, hipMemcpyHostToDevice));
-        scratch = std::max(scratch, (size_t)2 * g.count * (elem_bytes(g.field) << g.log2n));
+    }
+    // Lanes.  Work estimate of a transform: n log2 n butterflies' worth,
+    // Goldilocks words 2.5 times a 32-bit word's.
+    auto work = [](uint32_t field, uint32_t L) -> uint64_t {
+        return ((uint64_t)(field == STNTT_FIELD_GOLDILOCKS ? 5 : 2) << L) * std::max<uint32_t>(L, 1);
+    };
+    uint64_t queue_work = 0;
+    for (const BatchTask &t : tasks) queue_work += work(t.field, t.log_n);
+    std::vector<uint64_t> gwork(b->groups.size());
+    for (size_t gi = 0; gi < b->groups.size(); ++gi)
+        gwork[gi] = b->groups[gi].count * work(b->groups[gi].field, b->groups[gi].log2n);
+    b->launch_order.resize(b->groups.size());
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) b->launch_order[gi] = gi;
+    std::stable_sort(b->launch_order.begin(), b->launch_order.end(),
+                     [&](size_t x, size_t y) { return gwork[x] > gwork[y]; });
+    const size_t units = b->groups.size() + (b->count ? 1 : 0);
+    b->lanes = (int)std::max<size_t>(1, std::min<size_t>(stntt_batch::MAX_LANES, units));
+    uint64_t load[stntt_batch::MAX_LANES] = {};
+    auto least = [&]() {
+        int l = 0;
+        for (int i = 1; i < b->lanes; ++i)
+            if (load[i] < load[l]) l = i;
+        return l;
+    };
+    size_t lane_need[stntt_batch::MAX_LANES] = {};
+    b->group_lane.assign(b->groups.size(), 0);
+    bool queue_placed = !b->count;
+    for (size_t k = 0; k < b->launch_order.size() || !queue_placed; ++k) {
+        const size_t gi = k < b->launch_order.size() ? b->launch_order[k] : 0;
+        if (!queue_placed && (k >= b->launch_order.size() || queue_work >= gwork[gi])) {
+            b->queue_lane = least();
+            load[b->queue_lane] += queue_work;
+            queue_placed = true;
+        }
+        if (k >= b->launch_order.size()) break;
+        const int l = least();
+        b->group_lane[gi] = l;
+        load[l] += gwork[gi];
+        // a ping-pong of the group's words: the most any pass schedule of
+        // the group needs (splits may be re-tuned after the batch is built)
+        const BatchGroup &g = b->groups[gi];
+        lane_need[l] = std::max(lane_need[l], (size_t)2 * g.count * (elem_bytes(g.field) << g.log2n));
+    }
+    size_t scratch = 0;
+    for (int l = 0; l < b->lanes; ++l) {
+        b->lane_off[l] = scratch;
+        scratch += (lane_need[l] + 255) & ~(size_t)255;
     }
     if (scratch) HIP_TRY(hipMalloc(&b->d_group_scratch, scratch));
+    if (b->lanes > 1) {
+        HIP_TRY(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
+        for (int l = 1; l < b->lanes; ++l) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming));
+        }
+    }
     b->descs.assign(descs, descs + count);
     return STNTT_OK;
 }
 
 static int batch_launch(stntt_batch *b, hipStream_t s) {
-    launch_batch_reset(b->d_counter, s);
-    HIP_TRY(hipGetLastError());
+    hipStream_t ls[stntt_batch::MAX_LANES] = {s};
+    for (int l = 1; l < b->lanes; ++l) ls[l] = b->lane_stream[l];
+    if (b->lanes > 1) {  // fork: every lane starts after what s holds
+        HIP_TRY(hipEventRecord(b->fork_ev, s));
+        for (int l = 1; l < b->lanes; ++l) HIP_TRY(hipStreamWaitEvent(ls[l], b->fork_ev, 0));
+    }
     if (b->count) {
-        launch_batch(b->grid, b->d_tasks, b->count, b->d_counter, b->d_scratch, b->scratch_stride, s);
+        hipStream_t q = ls[b->queue_lane];
+        launch_batch_reset(b->d_counter, q);
+        HIP_TRY(hipGetLastError());
+        launch_batch(b->grid, b->d_tasks, b->count, b->d_counter, b->d_scratch, b->scratch_stride, q);
         HIP_TRY(hipGetLastError());
     }
-    for (const BatchGroup &g : b->groups) {
+    for (size_t gi : b->launch_order) {
+        const BatchGroup &g = b->groups[gi];
+        const int l = b->group_lane[gi];
+        void *scratch = (char *)b->d_group_scratch + b->lane_off[l];
         int rc;
         if (g.field == STNTT_FIELD_GOLDILOCKS)
-            rc = run_transform<FieldGL>(b->ctx, *g.plan, nullptr, g.count, g.flags, s, nullptr,
+            rc = run_transform<FieldGL>(b->ctx, *g.plan, nullptr, g.count, g.flags, ls[l], nullptr,
                                         (const uint64_t *const *)g.d_src, (uint64_t *const *)g.d_dst,
-                                        (uint64_t *)b->d_group_scratch);
+                                        (uint64_t *)scratch);
         else
-            rc = run_transform<FieldU32>(b->ctx, *g.plan, nullptr, g.count, g.flags, s, nullptr,
+            rc = run_transform<FieldU32>(b->ctx, *g.plan, nullptr, g.count, g.flags, ls[l], nullptr,
                                          (const uint32_t *const *)g.d_src, (uint32_t *const *)g.d_dst,
-                                         (uint32_t *)b->d_group_scratch);
+                                         (uint32_t *)scratch);
         if (rc) return rc;
+    }
+    for (int l = 1; l < b->lanes; ++l) {  // join: s continues after every lane
+        HIP_TRY(hipEventRecord(b->join_ev[l], ls[l]));
+        HIP_TRY(hipStreamWaitEvent(s, b->join_ev[l], 0));
     }
     return STNTT_OK;
 }
 
 static void batch_free(stntt_batch *b) {
     if (b->run_ev) (void)hipEventDestroy(b->run_ev);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    for (int l = 1; l < stntt_batch::MAX_LANES; ++l) {
+        if (b->join_ev[l]) (void)hipEventDestroy(b->join_ev[l]);
+        if (b->lane_stream[l]) (void)hipStreamDestroy(b->lane_stream[l]);
+    }
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tasks) (void)hipFree(b->d_tasks);
