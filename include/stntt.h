@@ -236,10 +236,14 @@ typedef struct stntt_batch stntt_batch;
 /* Validates every descriptor, makes the twiddle plans resident, orders the
    queue and uploads it.  The descriptors' buffers must stay valid. */
 int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out);
-/* the persistent launch (preceded by a one-thread queue reset kernel), then
-   the groups' pass launches, on one stream.  Runs of one batch share its
-   queue and scratch: a run (or graph replay) on another stream than the
-   previous one waits for it with an event.  destroy waits for the last run. */
+/* the persistent launch (preceded by a one-thread queue reset kernel) and
+   the groups' pass launches, on up to four streams at once: `stream` and
+   three of the batch's own, which start after what `stream` holds and which
+   `stream` waits for before anything enqueued on it later (events; under
+   capture they become parallel branches of the graph).  Runs of one batch
+   share its queue and scratch: a run (or graph replay) on another stream
+   than the previous one waits for it with an event.  destroy waits for the
+   last run. */
 int stntt_batch_run(stntt_batch *b, stntt_stream stream);
 /* capture the launch into a HIP graph once; later runs replay it */
 int stntt_batch_capture(stntt_batch *b, stntt_stream stream);
