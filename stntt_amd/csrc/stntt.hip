@@ -921,12 +921,16 @@ static int batch_launch(stntt_batch *b, hipStream_t s) {
         HIP_TRY(hipEventRecord(b->fork_ev, s));
         for (int l = 1; l < b->lanes; ++l) HIP_TRY(hipStreamWaitEvent(ls[l], b->fork_ev, 0));
     }
-    if (b->count) {
+    if (b->count || b->groups.empty()) {
+        // (an empty batch still enqueues the reset, so a capture of it holds
+        // a node)
         hipStream_t q = ls[b->queue_lane];
         launch_batch_reset(b->d_counter, q);
         HIP_TRY(hipGetLastError());
-        launch_batch(b->grid, b->d_tasks, b->count, b->d_counter, b->d_scratch, b->scratch_stride, q);
-        HIP_TRY(hipGetLastError());
+        if (b->count) {
+            launch_batch(b->grid, b->d_tasks, b->count, b->d_counter, b->d_scratch, b->scratch_stride, q);
+            HIP_TRY(hipGetLastError());
+        }
     }
     for (size_t gi : b->launch_order) {
         const BatchGroup &g = b->groups[gi];

@@ -215,6 +215,49 @@ def test_large_descriptors_in_a_batch(ctx):
     b.close()
 
 
+def test_empty_batch_runs_captures_and_replays(ctx):
+    b = S.Batch(ctx, [])
+    b.run()
+    b.capture()
+    b.run_graph()
+    b.run_graph()
+    ctx.sync()
+    b.close()
+
+
+def test_lanes_from_a_callers_stream(ctx):
+    """Two groups and queue work go out on three streams forked from the
+    caller's stream and joined back into it: eager runs and graph replays
+    on that stream, every result exact after each."""
+    class Spec:
+        def __init__(self, field, inverse, log2n):
+            self.field, self.inverse, self.log2n = field, inverse, log2n
+    specs = [Spec("goldilocks", False, 16), Spec("u32", True, 15), Spec("goldilocks", False, 16),
+             Spec("u32", True, 15), Spec("goldilocks", False, 11), Spec("u32", False, 10), Spec("goldilocks", True, 12)]
+    inputs = make_inputs(specs, 77)
+    bufs = upload_all(ctx, inputs)
+    other = S.Context(0)
+    try:
+        st = other.stream
+        b = S.Batch(ctx, descs_of(specs, inputs, bufs))
+        b.run(st)
+        ctx.sync(st)
+        check_all(specs, inputs, bufs, 1, "eager")
+        b.capture(st)
+        b.run_graph(st)
+        ctx.sync(st)
+        check_all(specs, inputs, bufs, 2, "replay 1")
+        b.run(st)
+        b.run_graph(st)
+        ctx.sync(st)
+        check_all(specs, inputs, bufs, 4, "eager + replay 2")
+        b.close()
+    finally:
+        for x in bufs:
+            x.free()
+        other.close()
+
+
 def test_closing_the_context_releases_its_batches_and_buffers():
     """A batch or buffer may outlive an explicit Context.close(): the context
     releases them first, and their own close/free are then no-ops."""
