@@ -12,7 +12,9 @@
 //      by admitted requests stay within the HBM budget;
 //   2. launches each admission round: its uploads go on the upload stream in
 //      admission order (one transfer at a time at full PCIe bandwidth, not
-//      shared among workers); a free worker stream waits for them and runs
+//      shared among workers); the least busy worker with fewer than
+//      SERVER_WORKER_ROUNDS rounds in flight takes the round: its stream
+//      waits for the uploads and runs
 //      the round's small transforms (log2 n <= SERVER_BATCH_MAX_LOG) as one
 //      persistent descriptor-queue launch, then the larger ones through the
 //      multi-pass path, requests of one shape (field, size, direction,
@@ -51,6 +53,11 @@ namespace stntt {
 constexpr uint32_t SERVER_BATCH_MAX_LOG = 16;   // largest size sent through the batch kernel
 constexpr uint32_t SERVER_BATCH_GRID = 256;     // workgroups of one round's batch launch
 constexpr uint32_t SERVER_MAX_ROUND = 256;      // requests per admission round
+// rounds a worker holds in flight: the next round's uploads and launches are
+// enqueued while the previous one still computes or downloads (each round
+// in flight has its own host staging slot; device-side state is ordered by
+// the worker's stream)
+constexpr int SERVER_WORKER_ROUNDS = 2;
 
 using Clock = std::chrono::steady_clock;
 
@@ -69,12 +76,17 @@ struct ServerRequest {
 struct ServerWorker {
     stntt_ctx *ctx = nullptr;
     hipStream_t stream = nullptr;
-    BatchTask *h_tasks = nullptr;  // pinned staging of the round's descriptors
+    // pinned staging of a round's descriptors and grouped buffer pointers,
+    // SERVER_WORKER_ROUNDS slots of SERVER_MAX_ROUND entries: the host
+    // writes a slot before its copies are enqueued, so a slot stays with its
+    // round until the round retires
+    BatchTask *h_tasks = nullptr;
+    void **h_tab = nullptr;
+    bool slot_busy[SERVER_WORKER_ROUNDS] = {};
     BatchTask *d_tasks = nullptr;
     uint32_t *d_counter = nullptr;
     uint64_t *d_scratch = nullptr;  // SERVER_BATCH_GRID slices of 2^SERVER_BATCH_MAX_LOG words
-    void **h_tab = nullptr;         // pinned staging of the round's grouped buffer pointers
-    void **d_tab = nullptr;         // their device copy (in and out tables of grouped runs)
+    void **d_tab = nullptr;         // device copy of a round's pointer tables (in and out tables of grouped runs)
     int inflight = 0;
 };
 
@@ -87,6 +99,7 @@ struct ServerGroup {
 
 struct ServerRound {
     int worker;
+    int slot;  // the worker's staging slot
     std::vector<ServerGroup> groups;
     std::vector<hipEvent_t> aux;  // ordering events, recycled when the round retires
     size_t pending = 0;           // groups not yet retired

@@ -1076,6 +1076,8 @@ static uint64_t server_chunk(const stntt_ctx *ctx, int field, uint32_t L, uint64
 static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest *> &round, ServerRound &rd) {
     ServerWorker &W = S->workers[w];
     hipStream_t s = W.stream;
+    BatchTask *const h_tasks = W.h_tasks + (size_t)rd.slot * SERVER_MAX_ROUND;
+    void **const h_tab = W.h_tab + (size_t)rd.slot * SERVER_MAX_ROUND;
     auto ev = [&](std::vector<hipEvent_t> &into) -> hipEvent_t {
         hipEvent_t e = server_event(S);
         if (e) into.push_back(e);
@@ -1097,7 +1099,7 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
     for (ServerRequest *r : round) {
         const stntt_request &q = r->req;
         if (q.log2n > SERVER_BATCH_MAX_LOG) continue;
-        BatchTask &t = W.h_tasks[nb++];
+        BatchTask &t = h_tasks[nb++];
         t = BatchTask{};
         const Plan *plan;
         int rc;
@@ -1137,7 +1139,7 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
         return STNTT_OK;
     };
     if (nb) {
-        HIP_TRY(hipMemcpyAsync(W.d_tasks, W.h_tasks, nb * sizeof(BatchTask), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(W.d_tasks, h_tasks, nb * sizeof(BatchTask), hipMemcpyHostToDevice, s));
         launch_batch_reset(W.d_counter, s);
         launch_batch(std::min<uint32_t>(nb, SERVER_BATCH_GRID), W.d_tasks, nb, W.d_counter, W.d_scratch,
                      (uint64_t)1 << SERVER_BATCH_MAX_LOG, s);
@@ -1187,8 +1189,8 @@ static int server_launch_round(stntt_server *S, int w, std::vector<ServerRequest
                                         s);
             } else {
                 void **tab = W.d_tab + ntab;
-                for (uint64_t j = 0; j < chunk; ++j) W.h_tab[ntab + j] = part[j]->d_buf;
-                HIP_TRY(hipMemcpyAsync(tab, W.h_tab + ntab, chunk * sizeof(void *), hipMemcpyHostToDevice, s));
+                for (uint64_t j = 0; j < chunk; ++j) h_tab[ntab + j] = part[j]->d_buf;
+                HIP_TRY(hipMemcpyAsync(tab, h_tab + ntab, chunk * sizeof(void *), hipMemcpyHostToDevice, s));
                 ntab += (uint32_t)chunk;
                 const Plan *plan;
                 if ((rc = get_plan(W.ctx, std::get<0>(k), q0.log2n, std::get<4>(k), std::get<5>(k), q0.direction,
@@ -1239,6 +1241,7 @@ static void server_loop(stntt_server *S) {
             }
             if (it->pending == 0) {
                 S->workers[it->worker].inflight--;
+                S->workers[it->worker].slot_busy[it->slot] = false;
                 for (hipEvent_t e : it->aux) S->free_events.push_back(e);
                 it = S->rounds.erase(it);
             } else {
@@ -1248,13 +1251,12 @@ static void server_loop(stntt_server *S) {
         if (progressed) S->cv_done.notify_all();
         if (S->stop && S->pending.empty() && S->rounds.empty()) break;
 
-        // 2. admit a round onto a free worker
+        // 2. admit a round onto the least busy worker with a round slot free
         int w = -1;
         for (size_t i = 0; i < S->workers.size(); ++i)
-            if (S->workers[i].inflight == 0) {
+            if (S->workers[i].inflight < SERVER_WORKER_ROUNDS &&
+                (w < 0 || S->workers[i].inflight < S->workers[w].inflight))
                 w = (int)i;
-                break;
-            }
         std::vector<ServerRequest *> round;
         if (w >= 0) {
             while (!S->pending.empty() && round.size() < SERVER_MAX_ROUND) {
@@ -1279,10 +1281,14 @@ static void server_loop(stntt_server *S) {
             }
         }
         if (!round.empty()) {
-            S->workers[w].inflight++;
+            ServerWorker &W = S->workers[w];
+            int slot = 0;
+            while (W.slot_busy[slot]) ++slot;  // inflight < SERVER_WORKER_ROUNDS: one is free
+            W.slot_busy[slot] = true;
+            W.inflight++;
             S->rounds_launched++;
             for (ServerRequest *r : round) S->batched_requests += r->req.log2n <= SERVER_BATCH_MAX_LOG ? 1 : 0;
-            ServerRound rd{w, {}, {}, 0};
+            ServerRound rd{w, slot, {}, {}, 0};
             lk.unlock();
             int rc = server_launch_round(S, w, round, rd);
             lk.lock();
@@ -1296,6 +1302,7 @@ static void server_loop(stntt_server *S) {
                     server_cache_put(S, *r);
                 }
                 S->workers[w].inflight--;
+                S->workers[w].slot_busy[slot] = false;
                 for (ServerGroup &g : rd.groups) S->free_events.push_back(g.ev);
                 for (hipEvent_t e : rd.aux) S->free_events.push_back(e);
                 S->cv_done.notify_all();
@@ -2018,11 +2025,11 @@ static int server_setup(stntt_server *S, int device, uint32_t workers) {
         // scratch for the largest served multi-pass transform, allocated now so
         // that no later growth synchronises the device
         if ((rc = ensure_buffer(&W.ctx->scratch, &W.ctx->scratch_bytes, 2 * (8ull << SERVER_MAX_LOG)))) return rc;
-        HIP_TRY(hipHostMalloc((void **)&W.h_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
+        HIP_TRY(hipHostMalloc((void **)&W.h_tasks, SERVER_WORKER_ROUNDS * SERVER_MAX_ROUND * sizeof(BatchTask)));
         HIP_TRY(hipMalloc(&W.d_tasks, SERVER_MAX_ROUND * sizeof(BatchTask)));
         HIP_TRY(hipMalloc(&W.d_counter, sizeof(uint32_t)));
         HIP_TRY(hipMalloc(&W.d_scratch, (size_t)SERVER_BATCH_GRID * (8ull << SERVER_BATCH_MAX_LOG)));
-        HIP_TRY(hipHostMalloc((void **)&W.h_tab, SERVER_MAX_ROUND * sizeof(void *)));
+        HIP_TRY(hipHostMalloc((void **)&W.h_tab, SERVER_WORKER_ROUNDS * SERVER_MAX_ROUND * sizeof(void *)));
         HIP_TRY(hipMalloc((void **)&W.d_tab, SERVER_MAX_ROUND * sizeof(void *)));
     }
     return STNTT_OK;
