@@ -135,6 +135,13 @@ struct stntt_ctx {
     hipStream_t scratch_last = nullptr;  // stream of that record (compared, never used)
     void *staging = nullptr;  // device buffer of the host-buffer entry points
     size_t staging_bytes = 0;
+    // A polynomial multiply's two forward transforms are independent: for
+    // small ones the second runs on `side`, forked from and joined back into
+    // the caller's stream, with a scratch of its own (created on first use).
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    void *side_scratch = nullptr;
+    size_t side_scratch_bytes = 0;
     // per-pass timing: when on, events bracket every kernel of a transform
     bool profile = false;
     static constexpr int MAX_PROFILE_EVENTS = 9;
@@ -579,9 +586,10 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
     bool src_caller = true;  // the pass reads the caller's buffers (else src)
     T *src = nullptr;
     uint32_t log_j = 0;
-    // per-pass events for a single caller's transform only (grouped runs of
-    // a batch go out on several streams at once)
-    const bool prof = ctx->profile && !in_tab && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
+    // per-pass events for a single caller's transform on the context's
+    // scratch only (grouped runs of a batch and a polynomial multiply's side
+    // transform go out on several streams at once)
+    const bool prof = ctx->profile && !in_tab && !ext_scratch && sch.npasses + 2 <= stntt_ctx::MAX_PROFILE_EVENTS;
     if (prof) {
         ctx->prof_count = 0;
         HIP_TRY(hipEventRecord(ctx->prof_ev[ctx->prof_count++], s));
@@ -1436,15 +1444,44 @@ static int launch_pointwise(typename FP::T *a, const typename FP::T *b, uint64_t
     return STNTT_OK;
 }
 
+// Polynomial multiplies of at most this many words per operand run their
+// two forward transforms on two streams: below it a transform leaves most
+// of the chip idle (Goldilocks 2^22: 132 -> 111 us for the pair), above it
+// the two compete for the same units (32-bit 2^24: 198 -> 213 us).
+constexpr uint64_t POLYMUL_SIDE_MAX_WORDS = 1ull << 22;
+
 // forward(a), forward(b) to bit-reversed order, then the inverse from
 // bit-reversed order with b multiplied in as the first pass loads a
 template <class FP>
 static int polymul_run(stntt_ctx *ctx, const Plan &fwd, const Plan &inv, typename FP::T *a, typename FP::T *b,
                        uint64_t batches, uint32_t flags, hipStream_t s) {
-    int rc = run_transform<FP>(ctx, fwd, a, batches, flags | STNTT_BITREV_OUT, s);
-    if (!rc) rc = run_transform<FP>(ctx, fwd, b, batches, flags | STNTT_BITREV_OUT, s);
-    if (!rc) rc = run_transform<FP>(ctx, inv, a, batches, STNTT_BITREV_IN, s, b);
-    return rc;
+    using T = typename FP::T;
+    const uint32_t L = fwd.key.log_n;
+    const uint32_t fflags = flags | STNTT_BITREV_OUT;
+    int rc;
+    if ((batches << L) <= POLYMUL_SIDE_MAX_WORDS) {
+        // the two forward transforms at once: b's on the side stream
+        const uint64_t need = schedule_scratch_bytes<FP>(
+            transform_schedule<FP>(ctx, fwd.key.field, L, batches, fflags), batches << L, fflags);
+        if (!ctx->side) {
+            HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming));
+        }
+        if (need && (rc = ensure_buffer(&ctx->side_scratch, &ctx->side_scratch_bytes, need))) return rc;
+        HIP_TRY(hipEventRecord(ctx->side_fork, s));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->side_fork, 0));
+        if ((rc = run_transform<FP>(ctx, fwd, a, batches, fflags, s))) return rc;
+        if ((rc = run_transform<FP>(ctx, fwd, b, batches, fflags, ctx->side, nullptr, nullptr, nullptr,
+                                    (T *)ctx->side_scratch)))
+            return rc;
+        HIP_TRY(hipEventRecord(ctx->side_join, ctx->side));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->side_join, 0));
+    } else {
+        if ((rc = run_transform<FP>(ctx, fwd, a, batches, fflags, s))) return rc;
+        if ((rc = run_transform<FP>(ctx, fwd, b, batches, fflags, s))) return rc;
+    }
+    return run_transform<FP>(ctx, inv, a, batches, STNTT_BITREV_IN, s, b);
 }
 
 extern "C" {
@@ -1493,6 +1530,10 @@ int stntt_ctx_destroy(stntt_ctx *ctx) {
     for (auto &kv : ctx->plans) (void)hipFree(kv.second->d_tables);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->staging) (void)hipFree(ctx->staging);
+    if (ctx->side_scratch) (void)hipFree(ctx->side_scratch);
+    if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
+    if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     for (auto &e : ctx->prof_ev)
