@@ -149,9 +149,12 @@ int stntt_pointwise_goldilocks(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_
 /* Cyclic (positive-wrapped) convolution of batches of length-2^log2n
    vectors, as pwcNtt (src/ntt/ntt_cpu.cpp:167-174): afterwards a holds
    a (*) b and b holds nttCt(b), its forward transform in bit-reversed order.
-   Three transform launches sequences: forward(a) and forward(b) with
+   Three transform launch sequences: forward(a) and forward(b) with
    bit-reversed output, then the inverse with bit-reversed input whose first
-   pass multiplies by b as it loads a (the pointwise product is fused).
+   pass multiplies by b as it loads a (the pointwise product is fused).  Up
+   to 2^22 words per operand, forward(b) runs on a side stream of the
+   context that starts after what `stream` holds and that `stream` waits for
+   before the inverse (events), so the two forward transforms overlap.
    flags: STNTT_SIGNED_INPUT (32-bit field) for int32 inputs. */
 int stntt_polymul_u32(stntt_ctx *ctx, uint32_t *d_a, uint32_t *d_b, uint32_t log2n, uint64_t batches,
                       uint32_t root, uint32_t mod, uint32_t flags, stntt_stream stream);
