@@ -57,7 +57,7 @@ constexpr uint32_t SERVER_MAX_ROUND = 256;      // requests per admission round
 // enqueued while the previous one still computes or downloads (each round
 // in flight has its own host staging slot; device-side state is ordered by
 // the worker's stream)
-constexpr int SERVER_WORKER_ROUNDS = 2;
+constexpr int SERVER_WORKER_ROUNDS = 3;
 
 using Clock = std::chrono::steady_clock;
 
