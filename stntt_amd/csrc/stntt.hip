@@ -1290,8 +1290,8 @@ static void server_loop(stntt_server *S) {
         }
         if (!round.empty()) {
             ServerWorker &W = S->workers[w];
-            int slot = 0;
-            while (W.slot_busy[slot]) ++slot;  // inflight < SERVER_WORKER_ROUNDS: one is free
+            int slot = 0;  // inflight < SERVER_WORKER_ROUNDS, so one slot is free
+            while (slot + 1 < SERVER_WORKER_ROUNDS && W.slot_busy[slot]) ++slot;
             W.slot_busy[slot] = true;
             W.inflight++;
             S->rounds_launched++;
