@@ -167,6 +167,51 @@ def test_polymul_goldilocks(ctx, L, batches):
         assert np.array_equal(got_b[sl], O.ntt_ct_gl(b[sl], L)), i
 
 
+def test_polymul_side_stream_limit_and_two_caller_streams(ctx):
+    """The largest multiplies whose forward pair overlaps (2^22 words per
+    operand: Goldilocks 2^21 x 2, 32-bit 64 x 2^16), issued back to back on
+    two callers' streams of one context (they share its side stream and its
+    scratch), then a repeat on the first stream; every result exact."""
+    other = S.Context(0)
+    try:
+        st = [ctx.stream, other.stream]
+        L1, B1 = 21, 2
+        n1 = 1 << L1
+        rng = np.random.default_rng(91)
+        a1 = rng.integers(0, P, size=B1 * n1, dtype=np.uint64)
+        b1 = rng.integers(0, P, size=B1 * n1, dtype=np.uint64)
+        L2, B2 = 16, 64
+        n2 = 1 << L2
+        root, mod = S.find_params(n2, 11)
+        a2 = rng.integers(0, mod, size=B2 * n2, dtype=np.uint64).astype(np.uint32)
+        b2 = rng.integers(0, mod, size=B2 * n2, dtype=np.uint64).astype(np.uint32)
+        da1, db1, da2, db2 = ctx.upload(a1), ctx.upload(b1), ctx.upload(a2), ctx.upload(b2)
+        ctx.polymul_goldilocks(da1, db1, L1, B1, stream=st[0])
+        ctx.polymul_u32(da2, db2, L2, B2, root, mod, stream=st[1])
+        ctx.sync(st[0])
+        ctx.sync(st[1])
+        want_a1 = np.concatenate([O.pwc_ntt_gl(a1[i * n1:(i + 1) * n1], b1[i * n1:(i + 1) * n1], L1)
+                                  for i in range(B1)])
+        want_b1 = np.concatenate([O.ntt_ct_gl(b1[i * n1:(i + 1) * n1], L1) for i in range(B1)])
+        want_a2 = np.concatenate([O.pwc_ntt_u32(a2[i * n2:(i + 1) * n2], b2[i * n2:(i + 1) * n2], root, mod)
+                                  for i in range(B2)])
+        want_b2 = np.concatenate([O.ntt_ct_u32(b2[i * n2:(i + 1) * n2], root, mod) for i in range(B2)])
+        assert np.array_equal(da1.copy_to(np.empty_like(a1)), want_a1)
+        assert np.array_equal(db1.copy_to(np.empty_like(b1)), want_b1)
+        assert np.array_equal(da2.copy_to(np.empty_like(a2)), want_a2)
+        assert np.array_equal(db2.copy_to(np.empty_like(b2)), want_b2)
+        # again on the first stream, from fresh inputs
+        da1.copy_from(a1)
+        db1.copy_from(b1)
+        ctx.polymul_goldilocks(da1, db1, L1, B1, stream=st[0])
+        ctx.sync(st[0])
+        assert np.array_equal(da1.copy_to(np.empty_like(a1)), want_a1)
+        for d in (da1, db1, da2, db2):
+            d.free()
+    finally:
+        other.close()
+
+
 def test_polymul_goldilocks_full_size_shift(ctx):
     """At 2^24, multiplying by z^s is a cyclic shift by s: a size-independent
     exact check of forward, fused pointwise product and inverse."""
