@@ -29,18 +29,20 @@ namespace stntt {
 constexpr uint64_t GL_P = 0xFFFFFFFF00000001ull;
 constexpr uint64_t GL_EPS = 0x00000000FFFFFFFFull;
 
-// any 64-bit word -> [0, P)
-GL_HD uint64_t gl_canon(uint64_t x) {
-    uint64_t t = x + GL_EPS;  // x - P (mod 2^64); wraps exactly when x >= P
-    return t < x ? t : x;
-}
+// x + EPS (mod 2^64) when c, else x: one 32-bit select and a 64-bit add of
+// the zero-extended word, where selecting between x and x + EPS takes a
+// select per half
+GL_HD uint64_t gl_add_eps_if(uint64_t x, bool c) { return x + (uint64_t)(c ? 0xFFFFFFFFu : 0u); }
 
-// a + b, a, b < P.  a + b < 2P: subtract P when the 64-bit sum carried or
-// reached P; s + EPS = s - P (mod 2^64) carries exactly in the second case.
+// any 64-bit word -> [0, P): x - P = x + EPS (mod 2^64) when x >= P
+GL_HD uint64_t gl_canon(uint64_t x) { return gl_add_eps_if(x, x >= GL_P); }
+
+// a + b, a, b < P.  a + b < 2P: subtract P when the 64-bit sum carried (the
+// 2^64 is worth EPS, and s + EPS < P then) or reached P (s + EPS = s - P
+// mod 2^64); both are s + EPS, and they never coincide.
 GL_HD uint64_t gl_add(uint64_t a, uint64_t b) {
     uint64_t s = a + b;
-    uint64_t t = s + GL_EPS;
-    return (s < a || t < s) ? t : s;
+    return gl_add_eps_if(s, (s < a) | (s >= GL_P));
 }
 
 // d = a - b (mod 2^64) as a two-word borrow chain; returns the borrow.  The
@@ -76,8 +78,7 @@ GL_HD uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
     uint64_t s;
     if (gl_sub_borrow(lo, h1, s)) s -= GL_EPS;
     const uint64_t r = h0 * GL_EPS + s;
-    const uint64_t t = r + GL_EPS;
-    return (r < s || t < r) ? t : r;
+    return gl_add_eps_if(r, (r < s) | (r >= GL_P));
 }
 
 // 64 x 64 -> 128 from four 32 x 32 -> 64 multiply-adds
@@ -106,8 +107,7 @@ GL_HD uint64_t gl_shl_small(uint64_t x, int s) {
     uint64_t lo = x << s;
     uint64_t h = x >> (64 - s);
     uint64_t r = h * GL_EPS + lo;
-    uint64_t t = r + GL_EPS;
-    return (r < lo || t < r) ? t : r;
+    return gl_add_eps_if(r, (r < lo) | (r >= GL_P));
 }
 
 // x * 2^S (mod P) for a compile-time S and any x; result in [0, P).
