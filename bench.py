@@ -42,20 +42,7 @@ import time
 
 import numpy as np
 
-# HIP gives a process GPU_MAX_HW_QUEUES hardware queues (4 unless set) and
-# maps further streams onto shared queues, whose commands then run in
-# submission order.  The serving leg's server keeps six streams busy (four
-# workers, one upload, one download) beside the contexts' own.  Measured
-# after the polynomial-multiply and batched legs (same box, two rounds):
-# 4 queues: server saturation 4.6-4.9 K transforms/s, sustains 1x its
-# sequential capacity, batch 1.49-1.54 ms; 8: 5.56-5.79 K/s, 1.5x, batch
-# 1.53-1.58 ms; 16: 5.74-5.83 K/s, 1.5x, but batch 1.70-1.71 ms.  HIP reads
-# the variable when it initialises, i.e. before the first HIP call.
-MIN_HW_QUEUES = 8
-if int(os.environ.get("GPU_MAX_HW_QUEUES") or 4) < MIN_HW_QUEUES:
-    os.environ["GPU_MAX_HW_QUEUES"] = str(MIN_HW_QUEUES)
-
-from stntt_amd import _lib  # noqa: E402
+from stntt_amd import _lib
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -591,7 +578,8 @@ def main() -> int:
         "hbm_frac_transform": round(alg_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "hbm_frac_transform_kernels": round(alg_bytes / (transform_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "clock": f"{args.settle_s} s of untimed transforms, then {args.warmup} warm-up steps",
-        "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+        # hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, 4 unless set)
+        "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES") or 4),
         # the bound that actually limits this kernel: VALU issue.  Wave64
         # instructions per launch (rocprofv3 SQ_INSTS_VALU, profiles/) over
         # the live mean duration, against 1024 SIMDs x 2.4 GHz / 4 cycles.

@@ -152,7 +152,7 @@ int stntt_pointwise_goldilocks(stntt_ctx *ctx, uint64_t *d_a, const uint64_t *d_
    Three transform launch sequences: forward(a) and forward(b) with
    bit-reversed output, then the inverse with bit-reversed input whose first
    pass multiplies by b as it loads a (the pointwise product is fused).  Up
-   to 2^22 words per operand, forward(b) runs on a side stream of the
+   to 2^22 words per operand, forward(b) runs on an auxiliary stream of the
    context that starts after what `stream` holds and that `stream` waits for
    before the inverse (events), so the two forward transforms overlap.
    flags: STNTT_SIGNED_INPUT (32-bit field) for int32 inputs. */
@@ -241,9 +241,11 @@ typedef struct stntt_batch stntt_batch;
 int stntt_batch_create(stntt_ctx *ctx, const stntt_desc *descs, size_t count, stntt_batch **out);
 /* the persistent launch (preceded by a one-thread queue reset kernel) and
    the groups' pass launches, on up to four streams at once: `stream` and
-   three of the batch's own, which start after what `stream` holds and which
-   `stream` waits for before anything enqueued on it later (events; under
-   capture they become parallel branches of the graph).  Runs of one batch
+   three auxiliary streams of the context (kept for its life, shared with
+   its other batches and its polynomial multiplies), which start after what
+   `stream` holds and which `stream` waits for before anything enqueued on
+   it later (events; under capture they become parallel branches of the
+   graph).  Runs of one batch
    share its queue and scratch: a run (or graph replay) on another stream
    than the previous one waits for it with an event.  destroy waits for the
    last run. */

@@ -135,10 +135,14 @@ struct stntt_ctx {
     hipStream_t scratch_last = nullptr;  // stream of that record (compared, never used)
     void *staging = nullptr;  // device buffer of the host-buffer entry points
     size_t staging_bytes = 0;
-    // A polynomial multiply's two forward transforms are independent: for
-    // small ones the second runs on `side`, forked from and joined back into
-    // the caller's stream, with a scratch of its own (created on first use).
-    hipStream_t side = nullptr;
+    // Auxiliary streams, created on first use and kept for the context's
+    // life (HIP maps a process's streams onto a few hardware queues, and
+    // streams created and destroyed per call were measured to land on busy
+    // ones): a polynomial multiply's side transform runs on aux[0], a
+    // batch's lanes 1..3 on aux[0..2].  Work on them is forked from and
+    // joined back into the caller's stream by events.
+    static constexpr int MAX_AUX = 3;
+    hipStream_t aux[MAX_AUX] = {};
     hipEvent_t side_fork = nullptr, side_join = nullptr;
     void *side_scratch = nullptr;
     size_t side_scratch_bytes = 0;
@@ -167,13 +171,14 @@ struct stntt_batch {
     uint32_t grid = 0;
     // The groups and the queue launch are independent of each other: they
     // run on up to MAX_LANES streams at once, lane 0 being the run's own
-    // stream and the others the batch's, forked from it and joined back into
-    // it by events (captured as parallel branches of the graph).  Work is
-    // dealt to lanes largest first, each to the least loaded lane; each lane
-    // has its own ping-pong scratch, at lane_off[lane] in d_group_scratch.
+    // stream and lanes 1.. the context's auxiliary streams, forked from it
+    // and joined back into it by events (captured as parallel branches of
+    // the graph).  Work is dealt to lanes largest first, each to the least
+    // loaded lane; each lane has its own ping-pong scratch, at
+    // lane_off[lane] in d_group_scratch.
     static constexpr int MAX_LANES = 4;
+    static_assert(MAX_LANES - 1 <= stntt_ctx::MAX_AUX, "a lane per auxiliary stream");
     int lanes = 1;
-    hipStream_t lane_stream[MAX_LANES] = {};  // [0] unused: the run's stream
     hipEvent_t fork_ev = nullptr;
     hipEvent_t join_ev[MAX_LANES] = {};
     int queue_lane = 0;                  // lane of the queue launch
@@ -192,6 +197,13 @@ namespace stntt {
 
 static int set_device(stntt_ctx *ctx) {
     HIP_TRY(hipSetDevice(ctx->device));
+    return STNTT_OK;
+}
+
+// the context's first `count` auxiliary streams, created on first use
+static int ctx_aux_streams(stntt_ctx *ctx, int count) {
+    for (int i = 0; i < count && i < stntt_ctx::MAX_AUX; ++i)
+        if (!ctx->aux[i]) HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking));
     return STNTT_OK;
 }
 
@@ -912,11 +924,9 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
     }
     if (scratch) HIP_TRY(hipMalloc(&b->d_group_scratch, scratch));
     if (b->lanes > 1) {
+        if (int rc = ctx_aux_streams(ctx, b->lanes - 1)) return rc;
         HIP_TRY(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
-        for (int l = 1; l < b->lanes; ++l) {
-            HIP_TRY(hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming));
-        }
+        for (int l = 1; l < b->lanes; ++l) HIP_TRY(hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming));
     }
     b->descs.assign(descs, descs + count);
     return STNTT_OK;
@@ -924,7 +934,7 @@ static int batch_build(stntt_ctx *ctx, const stntt_desc *descs, size_t count, st
 
 static int batch_launch(stntt_batch *b, hipStream_t s) {
     hipStream_t ls[stntt_batch::MAX_LANES] = {s};
-    for (int l = 1; l < b->lanes; ++l) ls[l] = b->lane_stream[l];
+    for (int l = 1; l < b->lanes; ++l) ls[l] = b->ctx->aux[l - 1];
     if (b->lanes > 1) {  // fork: every lane starts after what s holds
         HIP_TRY(hipEventRecord(b->fork_ev, s));
         for (int l = 1; l < b->lanes; ++l) HIP_TRY(hipStreamWaitEvent(ls[l], b->fork_ev, 0));
@@ -965,10 +975,8 @@ static int batch_launch(stntt_batch *b, hipStream_t s) {
 static void batch_free(stntt_batch *b) {
     if (b->run_ev) (void)hipEventDestroy(b->run_ev);
     if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
-    for (int l = 1; l < stntt_batch::MAX_LANES; ++l) {
+    for (int l = 1; l < stntt_batch::MAX_LANES; ++l)
         if (b->join_ev[l]) (void)hipEventDestroy(b->join_ev[l]);
-        if (b->lane_stream[l]) (void)hipStreamDestroy(b->lane_stream[l]);
-    }
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_tasks) (void)hipFree(b->d_tasks);
@@ -1463,19 +1471,20 @@ static int polymul_run(stntt_ctx *ctx, const Plan &fwd, const Plan &inv, typenam
         // the two forward transforms at once: b's on the side stream
         const uint64_t need = schedule_scratch_bytes<FP>(
             transform_schedule<FP>(ctx, fwd.key.field, L, batches, fflags), batches << L, fflags);
-        if (!ctx->side) {
-            HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        if ((rc = ctx_aux_streams(ctx, 1))) return rc;
+        if (!ctx->side_fork) {
             HIP_TRY(hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming));
             HIP_TRY(hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming));
         }
+        hipStream_t side = ctx->aux[0];
         if (need && (rc = ensure_buffer(&ctx->side_scratch, &ctx->side_scratch_bytes, need))) return rc;
         HIP_TRY(hipEventRecord(ctx->side_fork, s));
-        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->side_fork, 0));
+        HIP_TRY(hipStreamWaitEvent(side, ctx->side_fork, 0));
         if ((rc = run_transform<FP>(ctx, fwd, a, batches, fflags, s))) return rc;
-        if ((rc = run_transform<FP>(ctx, fwd, b, batches, fflags, ctx->side, nullptr, nullptr, nullptr,
+        if ((rc = run_transform<FP>(ctx, fwd, b, batches, fflags, side, nullptr, nullptr, nullptr,
                                     (T *)ctx->side_scratch)))
             return rc;
-        HIP_TRY(hipEventRecord(ctx->side_join, ctx->side));
+        HIP_TRY(hipEventRecord(ctx->side_join, side));
         HIP_TRY(hipStreamWaitEvent(s, ctx->side_join, 0));
     } else {
         if ((rc = run_transform<FP>(ctx, fwd, a, batches, fflags, s))) return rc;
@@ -1533,7 +1542,8 @@ int stntt_ctx_destroy(stntt_ctx *ctx) {
     if (ctx->side_scratch) (void)hipFree(ctx->side_scratch);
     if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
     if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    for (hipStream_t a : ctx->aux)
+        if (a) (void)hipStreamDestroy(a);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     for (auto &e : ctx->prof_ev)
