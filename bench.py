@@ -19,6 +19,10 @@ over four buffers in turn (512 MiB, beyond the 256 MB MALL, as the passes'
 ping-pong is): the HBM rate this footprint reaches in practice, with the
 dominant pass and the whole transform as fractions of it.
 
+Extra legs (--legs): n12 (Goldilocks 2^12, one transform), n20, gllarge,
+u32large, orders (element orders, polynomial multiply), field64, batched
+(4096 mixed transforms), serving (Poisson trace through the server).
+
 cpu_baseline: the oracle's restatement of the reference's CPU Stockham
 (src/ntt/ntt_cpu.cpp:46-90 with exact Goldilocks arithmetic), single
 thread, on a bounded sample of the same workload; the host's core count is
@@ -70,7 +74,7 @@ def parse():
                     help="also run a plain device copy of the same buffer (counter calibration)")
     ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--batch-count", type=int, default=4096)
-    ap.add_argument("--legs", default="n20,gllarge,u32large,orders,field64,batched,serving",
+    ap.add_argument("--legs", default="n12,n20,gllarge,u32large,orders,field64,batched,serving",
                     help="comma-separated extra legs to run after the headline")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched leg without the per-transform plain comparison")
@@ -106,6 +110,19 @@ def cpu_baseline(log2n: int, budget_s: float) -> dict:
     return {"value": n / med / 1e9, "unit": "GElem/s", "cores": 1, "host_cores": host_cores(), "kind": "port",
             "sample": f"{len(times)} whole Goldilocks 2^{log2n} forward transforms, median "
                       f"{med * 1e3:.0f} ms each, single thread (oracle/ntt_oracle.c)"}
+
+
+def cpu_small(log2n: int, budget_s: float = 0.2) -> dict:
+    """The oracle on one Goldilocks 2^log2n transform, single thread: mean
+    over at least 20 transforms and about budget_s."""
+    from oracle import oracle as O
+    x = np.random.default_rng(12).integers(0, P, size=1 << log2n, dtype=np.uint64)
+    reps, t0 = 0, time.perf_counter()
+    while reps < 20 or time.perf_counter() - t0 < budget_s:
+        O.stockham_gl(x, log2n)
+        reps += 1
+    return {"forward_us": round((time.perf_counter() - t0) / reps * 1e6, 1), "cores": 1, "kind": "port",
+            "sample": f"{reps} Goldilocks 2^{log2n} forward transforms, single thread (oracle/ntt_oracle.c)"}
 
 
 def cpu_all_cores(specs, budget_s: float, label: str) -> dict:
@@ -192,6 +209,27 @@ def bench_n20(S, ctx, steps: int) -> dict:
                       "gelem_s": round(2 * n / dt / 1e9, 3)}
         buf.free()
     return out
+
+
+def bench_n12(S, ctx, steps: int) -> dict:
+    """Goldilocks 2^12, one transform (the reference's own CPU-runnable
+    size): device-resident forward and forward + inverse latency (the CPU
+    leg times the oracle on the same transform: cpu_n2^12)."""
+    n = 1 << 12
+    x = np.random.default_rng(12).integers(0, P, size=n, dtype=np.uint64)
+    buf = ctx.upload(x)
+    fwd = lambda: ctx.ntt_goldilocks(buf, 12, 1)  # noqa: E731
+
+    def pair():
+        ctx.ntt_goldilocks(buf, 12, 1)
+        ctx.ntt_goldilocks(buf, 12, 1, inverse=True)
+    tf, nf = _time_steps(ctx, fwd, steps)
+    for _ in range(nf):  # undo the forward transforms
+        ctx.ntt_goldilocks(buf, 12, 1, inverse=True)
+    tp, _ = _time_steps(ctx, pair, steps)
+    ok = np.array_equal(buf.copy_to(np.empty_like(x)), x)
+    buf.free()
+    return {"forward_us": round(tf * 1e6, 2), "fwd_inv_us": round(tp * 1e6, 2), "roundtrip_exact": bool(ok)}
 
 
 def _time_steps(ctx, step, steps: int, settle_s: float = 0.08) -> tuple[float, int]:
@@ -597,6 +635,9 @@ def main() -> int:
                        "dominant_pass_frac_of_copy": round(achieved / 1e3 / copy_tbs, 4),
                        "transform_frac_of_copy": round(npass * alg_bytes / (transform_ms * 1e-3) / 1e12 / copy_tbs, 4)}
     legs = set() if args.no_extra else set(args.legs.split(","))
+    if "n12" in legs:
+        log("[bench] n=2^12 Goldilocks, one transform")
+        out["n2^12_goldilocks"] = bench_n12(S, ctx, 200)
     if "n20" in legs:
         log("[bench] n=2^20 forward+inverse, both fields")
         out["n2^20_fwd_inv"] = bench_n20(S, ctx, 20)
@@ -621,6 +662,8 @@ def main() -> int:
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)
+        if "n12" in legs:
+            out["cpu_n2^12"] = cpu_small(12)
         if "batched" in legs or "serving" in legs:
             from stntt_amd.workloads import mixed_batch, poisson_arrivals
             log("[bench] cpu, all cores: batched and serving workloads")

@@ -87,7 +87,13 @@ template <class FP>
 inline Schedule make_schedule(int L, uint64_t batches, bool allow_big = true) {
     Schedule s;
     const bool fits_one = L <= MAX_LOGR_SINGLE<FP>;
-    if (L < min_tile_log<FP>() || (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
+    // Sizes up to NARROW_MAX_LOGR always run in one pass: below the two-pass
+    // tile minimum there is no other schedule, and for 8-byte words at 2^8 and
+    // 2^9 one pass beat two tile passes (a lone Goldilocks 2^9: 7.1 against
+    // 10.3 us, 2^8: 8.6 against 9.0 us; the 32-bit field's tile minimum is
+    // 2^10 already).
+    if (L < min_tile_log<FP>() || L <= NARROW_MAX_LOGR ||
+        (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
         s.single = true;
         s.npasses = 1;
         s.logr[0] = L;
