@@ -16,7 +16,7 @@ namespace stntt {
 template <class FP, int DIR, int LOGR>
 static void launch_single(const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s) {
     constexpr int LOGC = pass_logc<FP>(LOGR, true);
-    if constexpr (LOGR <= NARROW_MAX_LOGR) {
+    if constexpr (LOGR <= narrow_max_logr<FP>()) {
         if (narrow) {  // one transform, one wave
             hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, 0, NARROW_THREADS, DIR, true>), dim3(grid),
                                dim3(NARROW_THREADS), 0, s, a);

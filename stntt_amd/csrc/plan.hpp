@@ -22,7 +22,14 @@ constexpr int MIN_LOGR_TILE = 4;    // smallest radix of a tile pass
 constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile)
 constexpr int MIN_LOGR_BIG = 10;    // passes of radix 2^10..2^12 run the large-radix kernel
 constexpr int MAX_LOGR_BIG = 12;
-constexpr int NARROW_MAX_LOGR = 9;  // single-pass sizes with a one-transform, one-wave variant
+// single-pass sizes with a one-transform, one-wave variant: 2^9 for 8-byte
+// words, 2^10 for 4-byte words (a lone 32-bit 2^10 transform: 8.1 us in one
+// wave against 9.3 us in two tile passes; Goldilocks 2^10: 13.2 against
+// 10.7 us, and 2^11 one-wave transforms lost in both fields)
+template <class FP>
+constexpr int narrow_max_logr() {
+    return sizeof(typename FP::T) == 4 ? 10 : 9;
+}
 constexpr int NARROW_THREADS = 64;
 
 // log2 of the tile size in elements: 64 KiB of LDS per workgroup
@@ -87,12 +94,11 @@ template <class FP>
 inline Schedule make_schedule(int L, uint64_t batches, bool allow_big = true) {
     Schedule s;
     const bool fits_one = L <= MAX_LOGR_SINGLE<FP>;
-    // Sizes up to NARROW_MAX_LOGR always run in one pass: below the two-pass
-    // tile minimum there is no other schedule, and for 8-byte words at 2^8 and
-    // 2^9 one pass beat two tile passes (a lone Goldilocks 2^9: 7.1 against
-    // 10.3 us, 2^8: 8.6 against 9.0 us; the 32-bit field's tile minimum is
-    // 2^10 already).
-    if (L < min_tile_log<FP>() || L <= NARROW_MAX_LOGR ||
+    // Sizes up to narrow_max_logr always run in one pass: below the two-pass
+    // tile minimum there is no other schedule, and above it one pass beat
+    // two tile passes (a lone Goldilocks 2^9: 7.1 against 10.3 us, 2^8: 8.6
+    // against 9.0 us; 32-bit 2^10: 8.1 against 9.3 us).
+    if (L < min_tile_log<FP>() || L <= narrow_max_logr<FP>() ||
         (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
         s.single = true;
         s.npasses = 1;
