@@ -21,7 +21,7 @@ namespace stntt {
 
 // One pass of a transform schedule (see plan.hpp):
 //   single: the whole transform in one LDS pass (R = n), `narrow` selecting
-//           the one-transform, one-wave variant;
+//           the one-transform-per-workgroup variant;
 //   otherwise a tile pass of radix 2^logr; `first` (J = 1) / last pass are
 //           told apart by `first` and PassArgs::flags; `perm` stages a
 //           bit-reversed input (first pass) or output (last pass).

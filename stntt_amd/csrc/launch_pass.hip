@@ -17,9 +17,9 @@ template <class FP, int DIR, int LOGR>
 static void launch_single(const PassArgs<FP> &a, uint32_t grid, bool narrow, hipStream_t s) {
     constexpr int LOGC = pass_logc<FP>(LOGR, true);
     if constexpr (LOGR <= narrow_max_logr<FP>()) {
-        if (narrow) {  // one transform, one wave
-            hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, 0, NARROW_THREADS, DIR, true>), dim3(grid),
-                               dim3(NARROW_THREADS), 0, s, a);
+        if (narrow) {  // one transform per workgroup
+            constexpr int NT = narrow_threads<FP, LOGR>();
+            hipLaunchKernelGGL((ntt_pass_kernel<FP, LOGR, 0, NT, DIR, true>), dim3(grid), dim3(NT), 0, s, a);
             return;
         }
     }

@@ -22,15 +22,23 @@ constexpr int MIN_LOGR_TILE = 4;    // smallest radix of a tile pass
 constexpr int MAX_LOGR_TILE = 9;    // largest radix of a tile pass (64 KiB tile)
 constexpr int MIN_LOGR_BIG = 10;    // passes of radix 2^10..2^12 run the large-radix kernel
 constexpr int MAX_LOGR_BIG = 12;
-// single-pass sizes with a one-transform, one-wave variant: 2^9 for 8-byte
-// words, 2^10 for 4-byte words (a lone 32-bit 2^10 transform: 8.1 us in one
-// wave against 9.3 us in two tile passes; Goldilocks 2^10: 13.2 against
-// 10.7 us, and 2^11 one-wave transforms lost in both fields)
+// Single-pass sizes with a one-transform variant (a workgroup of
+// narrow_threads per transform): up to 2^10 for 8-byte words, 2^12 for
+// 4-byte words.  A lone transform in one pass against two tile passes:
+// Goldilocks 2^9 7.0 against 10.3 us, 2^10 9.7 against 10.7 us (2^11 12.2
+// against 11.4, 2^12 13.9 against 11.4: two passes win); 32-bit 2^10 8.1
+// against 9.3 us, 2^11 8.8 against 10.7, 2^12 9.5 against 9.8 us.
 template <class FP>
 constexpr int narrow_max_logr() {
-    return sizeof(typename FP::T) == 4 ? 10 : 9;
+    return sizeof(typename FP::T) == 4 ? 12 : 10;
 }
-constexpr int NARROW_THREADS = 64;
+// threads of a one-transform single pass: 8 words per thread for 8-byte
+// words, 16 for 4-byte words, at least one wave
+template <class FP, int LOGR>
+constexpr int narrow_threads() {
+    constexpr int ept = sizeof(typename FP::T) == 4 ? 16 : 8;
+    return (1 << LOGR) / ept < 64 ? 64 : (1 << LOGR) / ept;
+}
 
 // log2 of the tile size in elements: 64 KiB of LDS per workgroup
 template <class FP>
@@ -96,8 +104,7 @@ inline Schedule make_schedule(int L, uint64_t batches, bool allow_big = true) {
     const bool fits_one = L <= MAX_LOGR_SINGLE<FP>;
     // Sizes up to narrow_max_logr always run in one pass: below the two-pass
     // tile minimum there is no other schedule, and above it one pass beat
-    // two tile passes (a lone Goldilocks 2^9: 7.1 against 10.3 us, 2^8: 8.6
-    // against 9.0 us; 32-bit 2^10: 8.1 against 9.3 us).
+    // two tile passes (measured above).
     if (L < min_tile_log<FP>() || L <= narrow_max_logr<FP>() ||
         (fits_one && (batches << L) >= (1ull << 20) && batches >= 128)) {
         s.single = true;

@@ -638,7 +638,7 @@ static int run_transform(stntt_ctx *ctx, const Plan &plan, typename FP::T *d_dat
         if (sch.single) {
             uint64_t c = 1ull << pass_logc<FP>(logr, true);
             grid = (uint32_t)((batches + c - 1) / c);
-            // too few wide tiles to fill the chip: one transform per one-wave
+            // too few wide tiles to fill the chip: one transform per
             // workgroup instead (a wide tile computes all C columns)
             if (logr <= narrow_max_logr<FP>() && grid < 2 * (uint64_t)ctx->num_cus && c > 1) {
                 narrow = true;
