@@ -268,8 +268,10 @@ int stntt_set_batch_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t ns
    Every context starts with the splits tuned on MI355X (compiled into the
    library from stntt_amd/tuning/gfx950.json); this replaces one of them,
    and npasses = 0 removes it (the fewest-pass default then applies).  Each
-   pass covers 4..12 stages (10..12: the large-radix kernel).  Applies to single transforms (batches == 1,
-   what the autotuner times); batches keep the fewest-pass default. */
+   pass covers 4..12 stages (10..12: the large-radix kernel).  Applies to
+   single transforms (batches == 1, what the autotuner times) of sizes that
+   run in several passes; batches keep the fewest-pass default, and sizes
+   up to 2^10 (8-byte words) or 2^12 (32-bit words) always run in one pass. */
 int stntt_set_pass_split(stntt_ctx *ctx, int field, uint32_t log2n, uint32_t npasses, const uint32_t *logr);
 /* stntt_set_pass_split with the tile shape too: passes of radix 2^10..2^12
    run the large-radix kernel on tiles of R x 2^logc[i] words (the full
