@@ -38,6 +38,12 @@ def word_log2c(field: str) -> int:
     return 5 if field == "u32" else 4
 
 
+def single_pass_max_log(field: str) -> int:
+    """Largest log2 n the library always runs in one pass (csrc/plan.hpp,
+    narrow_max_logr): pass splits of these sizes are never used."""
+    return 12 if field == "u32" else 10
+
+
 def apply_table(ctx, path: str = TABLE) -> int:
     """Install a tuning table into ctx; returns the number of entries applied."""
     from .ntt import set_batch_split, set_pass_shape
@@ -178,7 +184,7 @@ def tune(logs_pass=range(8, 27), logs_batch=range(10, 19), verbose: bool = True,
         logc = word_log2c(field)
         for L in logs_pass:
             cands = pass_candidates(L, logc)
-            if not cands:  # below two tile passes: the single-pass kernel only
+            if not cands or L <= single_pass_max_log(field):  # the single-pass kernel only
                 continue
             n = 1 << L
             if field == "u32":

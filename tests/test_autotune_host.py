@@ -108,3 +108,15 @@ def test_measure_alternates_order_and_takes_medians():
     timed = order[3:]
     assert [timed[i] for i in range(0, len(timed), 2)] == names + names[::-1] + names
     assert all(timed[i] == timed[i + 1] for i in range(0, len(timed), 2))
+
+
+def test_single_pass_sizes_match_the_library():
+    """The tuner skips the sizes the library always runs in one pass; the
+    limits it assumes are the ones csrc/plan.hpp compiles in."""
+    import re
+    src = open(os.path.join(os.path.dirname(A.__file__), "csrc", "plan.hpp")).read()
+    m = re.search(r"constexpr int narrow_max_logr\(\) \{\s*return sizeof\(typename FP::T\) == 4 \? (\d+) : (\d+);", src)
+    assert m, "narrow_max_logr not found"
+    assert A.single_pass_max_log("u32") == int(m.group(1))
+    assert A.single_pass_max_log("goldilocks") == int(m.group(2))
+    assert A.single_pass_max_log("u64") == int(m.group(2))
