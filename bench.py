@@ -125,6 +125,33 @@ def cpu_small(log2n: int, budget_s: float = 0.2) -> dict:
             "sample": f"{reps} Goldilocks 2^{log2n} forward transforms, single thread (oracle/ntt_oracle.c)"}
 
 
+def cpu_reference_u32(log2n: int, budget_s: float = 2.0) -> dict | None:
+    """The reference's own CPU nttStockham (src/ntt/ntt_cpu.cpp:87-90,
+    compiled in place into oracle/_ref by oracle/Makefile) on one 32-bit
+    transform with findParams(n, 11), single thread, int32 data as it takes
+    them; None where that build is absent."""
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    n = 1 << log2n
+    root, mod = O.find_params(n, 11)
+    base = np.random.default_rng(20).integers(0, 10, size=n).astype(np.int32)
+    R = O.ref()
+    times = []
+    t_end = time.perf_counter() + budget_s
+    while len(times) < 3 or time.perf_counter() < t_end:
+        x = base.copy()
+        t0 = time.perf_counter()
+        rc = R.ref_ntt_stockham(x, n, root, mod, 1, -1)
+        times.append(time.perf_counter() - t0)
+        if rc != 0:
+            return None
+    med = statistics.median(times)
+    return {"value": n / med / 1e9, "unit": "GElem/s", "forward_ms": round(med * 1e3, 3), "cores": 1,
+            "kind": "reference", "sample": f"{len(times)} 32-bit 2^{log2n} forward transforms (root {root}, "
+                                         f"p {mod}), median, single thread (oracle/_ref: src/ntt/ntt_cpu.cpp)"}
+
+
 def cpu_all_cores(specs, budget_s: float, label: str) -> dict:
     """The oracle on a thread pool of every host core (the C oracle releases
     the GIL) over the first transforms of a workload until about budget_s:
@@ -664,6 +691,10 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(L, args.cpu_seconds)
         if "n12" in legs:
             out["cpu_n2^12"] = cpu_small(12)
+        if "n20" in legs:
+            ref20 = cpu_reference_u32(20)
+            if ref20 is not None:
+                out["cpu_reference_u32_2^20"] = ref20
         if "batched" in legs or "serving" in legs:
             from stntt_amd.workloads import mixed_batch, poisson_arrivals
             log("[bench] cpu, all cores: batched and serving workloads")
