@@ -110,6 +110,34 @@ GL_HD uint64_t gl_shl_small(uint64_t x, int s) {
     return gl_add_eps_if(r, (r < lo) | (r >= GL_P));
 }
 
+// x * 2^S (mod P) for 32 < S < 64 and any x; result in [0, P).  With
+// t = S - 32, x = xh 2^32 + xl and x 2^S = lo + hi 2^64: lo has a zero low
+// word and high word lh = xl 2^t mod 2^32; hi = h0 + h1 2^32 with
+// h0 = (x >> (32 - t)) mod 2^32 and h1 = xh >> (32 - t).  2^64 == 2^32 - 1
+// and 2^96 == -1 give
+//     x 2^S == lh 2^32 + h0 (2^32 - 1) - h1 = (lh + h0) 2^32 - (h0 + h1).
+// The carry c of lh + h0 = a + c 2^32 is worth 2^64 == 2^32 - 1, so
+//     x 2^S == (a + c) 2^32 - (h0 + h1 + c),
+// with a + c < 2^32 (a <= 2^32 - 2 when c = 1) and h0 + h1 + c < 2^33.  The
+// 64-bit difference is at most (2^32 - 1) 2^32 = P - 1 when it does not
+// borrow, and a borrow (a negative difference above -2^33) adds P.  Three
+// VALU fewer than the general 128-bit reduction of lo + hi 2^64.
+template <int S>
+GL_HD uint64_t gl_shl_mid(uint64_t x) {
+    static_assert(S > 32 && S < 64, "shift range");
+    constexpr int t = S - 32;
+    const uint32_t xh = (uint32_t)(x >> 32);
+    const uint32_t lh = (uint32_t)x << t;
+    const uint32_t h0 = (uint32_t)(x >> (32 - t));
+    const uint32_t h1 = xh >> (32 - t);
+    unsigned c, cb;
+    const uint32_t a = __builtin_addc(lh, h0, 0u, &c);
+    const uint32_t b0 = __builtin_addc(h0, h1, c, &cb);
+    uint64_t r;
+    const bool borrow = gl_sub_borrow((uint64_t)(a + c) << 32, ((uint64_t)cb << 32) | b0, r);
+    return borrow ? r + GL_P : r;
+}
+
 // x * 2^S (mod P) for a compile-time S and any x; result in [0, P).
 // 2^192 == 1 and 2^96 == -1.
 template <int S>
@@ -131,7 +159,7 @@ GL_HD uint64_t gl_mul_pow2(uint64_t x) {
         uint64_t r;
         if constexpr (s == 0) r = gl_canon(x);
         else if constexpr (s <= 32) r = gl_shl_small(x, s);
-        else r = gl_reduce128(x << s, x >> (64 - s));
+        else r = gl_shl_mid<s>(x);
         if constexpr (neg) return gl_sub(0, r);
         return r;
     }
