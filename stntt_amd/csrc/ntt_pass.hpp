@@ -102,6 +102,19 @@ __device__ __forceinline__ global_t<T> &at_off(T *base, uint32_t elem) {
     return *(global_t<T> *)((char *)base + (uint64_t)elem * sizeof(T));
 }
 
+// element at a 32-bit byte offset from a uniform base pointer: the base
+// stays a scalar register pair and the offset one 32-bit register
+// (global_load / global_store with saddr), where at_off's 64-bit offset
+// takes a 64-bit vector address per access.  For offsets below 2^32 only.
+template <class T>
+__device__ __forceinline__ const global_t<T> &at_b32(const T *base, uint32_t byte_off) {
+    return *(const global_t<T> *)((const char *)base + byte_off);
+}
+template <class T>
+__device__ __forceinline__ global_t<T> &at_b32(T *base, uint32_t byte_off) {
+    return *(global_t<T> *)((char *)base + byte_off);
+}
+
 // A word of a twiddle table through the constant address space (4): the
 // tables are never written while a transform runs, so a load at a uniform
 // address becomes a scalar load into SGPRs instead of a vector load that

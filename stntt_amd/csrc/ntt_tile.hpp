@@ -151,6 +151,15 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
     // than once per dependent load.  Row bases are uniform pointers; a lane
     // adds one 32-bit element offset.
     const uint64_t mstep = (uint64_t)(R / r1) * stride_in;
+    // 32-bit byte offsets from uniform row bases for 4-byte words.  A lane's
+    // offset within a row is below n / min(r1, r2) words (input lanes
+    // c + g n/R with g < R/r1; output rows g2 J with g2 < r1 and J <= n/R):
+    // < 2^30 bytes for n <= 2^30; twiddle table offsets are below 2^21 bytes
+    // (hi: 2^14 words, lo: n / 2^14).  The 32-bit field's 2^24 passes run
+    // 2-3 % faster with them; for 8-byte words at R >= 256 (where the
+    // offsets stay below 2^31 bytes up to n = 2^32) the Goldilocks passes
+    // measured about 1 % slower, so those keep 64-bit addresses.
+    constexpr bool OFF32 = sizeof(T) == 4;
     const auto roots = FP::template load_roots<LR1, DIR>(a.hi, a.log_rmax);
     const uint32_t lt = logn - LOGR;  // bits of a column index t
     if constexpr (PERM == PERM_IN) {
@@ -191,7 +200,7 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
         const uint32_t q = tid + gi * NT;
         const bool act = NG1 % NT == 0 || q < (uint32_t)NG1;
         const uint32_t c = q & (C - 1), g = q >> LOGC;
-        const uint32_t lane = t0 + c + g * stride_in;
+        [[maybe_unused]] const uint32_t lane = t0 + c + g * stride_in;
         const uint32_t tstep = g << (a.log_rmax - LOGR);  // w_R^(g k) = hi[k tstep]
         T v[r1];
         Tw w1[r1];
@@ -204,13 +213,21 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < r1; ++m) v[m] = at_off(in + (uint64_t)m * mstep, lane);
+                for (int m = 0; m < r1; ++m) {
+                    if constexpr (OFF32)
+                        v[m] = at_b32(in + t0 + (uint64_t)m * mstep, (c + g * stride_in) * (uint32_t)sizeof(T));
+                    else
+                        v[m] = at_off(in + (uint64_t)m * mstep, lane);
+                }
             }
             if constexpr (FP::CHEAP_TWIDDLE_PRODUCT) {
                 w1[1] = at_off(a.hi, tstep);  // the rest follow as its powers
             } else if constexpr (!INNER_LDS) {
 #pragma unroll
-                for (int k = 1; k < r1; ++k) w1[k] = at_off(a.hi, tstep * k);
+                for (int k = 1; k < r1; ++k) {
+                    if constexpr (OFF32) w1[k] = at_b32(a.hi, tstep * k * (uint32_t)sizeof(Tw));
+                    else w1[k] = at_off(a.hi, tstep * k);
+                }
             }
         }
         [[maybe_unused]] Tw ti[INNER_LDS ? R / NT : 1];
@@ -339,7 +356,8 @@ __device__ __forceinline__ void tile_body(const PassArgs<FP> &a, uint32_t block,
             } else {
                 T *orow = J1 ? out + (uint64_t)t0 * R + k2 * r1
                              : out + (uint64_t)sJ * R + (t0 - sJ) + ((uint64_t)(k2 * r1) << a.log_j);
-                at_off(orow, lane_out) = x;
+                if constexpr (OFF32) at_b32(orow, lane_out * (uint32_t)sizeof(T)) = x;
+                else at_off(orow, lane_out) = x;
             }
         }
         xc[gi] = c;
